@@ -1,0 +1,255 @@
+// Per-cell and per-face arithmetic of the Armon Lagrange + remap step, shared by the
+// per-stage drop-in kernels and the fused sweep kernels.
+//
+// Every expression keeps the operation order of the reference kernels (cited per
+// function) and the library is compiled with -ffp-contract=off, so that FP64 results
+// are bit-identical to an IEEE-strict CPU evaluation of the reference algorithm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define ARMON_HD __host__ __device__ __forceinline__
+
+namespace armon {
+
+// Kokkos::min / Kokkos::max on arithmetic types: (b < a ? b : a) and (a < b ? b : a).
+// These fix the sign of zero and the NaN behaviour, unlike fmin/fmax.
+template <class T> ARMON_HD T kmin(T a, T b) { return (b < a) ? b : a; }
+template <class T> ARMON_HD T kmax(T a, T b) { return (a < b) ? b : a; }
+
+// Correctly rounded y^3 and y^4 (Kokkos::pow(y, 3|4) is the correctly rounded
+// libm pow in the reference's IEEE build): double-double products made exact with
+// fma, rounded once.
+ARMON_HD double cr_pow3(double y)
+{
+    double p = y * y;
+    double e1 = __builtin_fma(y, y, -p);
+    double q = p * y;
+    double e2 = __builtin_fma(p, y, -q);
+    double t = e1 * y;
+    return q + (e2 + t);
+}
+
+ARMON_HD double cr_pow4(double y)
+{
+    double p = y * y;
+    double e1 = __builtin_fma(y, y, -p);
+    double q = p * p;
+    double e2 = __builtin_fma(p, p, -q);
+    double t = (2.0 * p) * e1;
+    return q + (e2 + t);
+}
+
+template <class T> ARMON_HD T sqrt_t(T x);
+template <> ARMON_HD double sqrt_t<double>(double x) { return __builtin_sqrt(x); }
+template <> ARMON_HD float sqrt_t<float>(float x) { return __builtin_sqrtf(x); }
+
+template <class T> ARMON_HD T copysign_t(T a, T b);
+template <> ARMON_HD double copysign_t<double>(double a, double b) { return __builtin_copysign(a, b); }
+template <> ARMON_HD float copysign_t<float>(float a, float b) { return __builtin_copysignf(a, b); }
+
+template <class T> ARMON_HD T abs_t(T a);
+template <> ARMON_HD double abs_t<double>(double a) { return __builtin_fabs(a); }
+template <> ARMON_HD float abs_t<float>(float a) { return __builtin_fabsf(a); }
+
+enum LimiterKind : int { LIM_NONE = 0, LIM_MINMOD = 1, LIM_SUPERBEE = 2 };
+
+// limiters.h:15-34
+template <int L, class T> ARMON_HD T limiter(T r)
+{
+    if constexpr (L == LIM_MINMOD) {
+        return kmax(T(0), kmin(T(1), r));
+    } else if constexpr (L == LIM_SUPERBEE) {
+        return kmax(kmax(T(0), kmin(T(1), 2 * r)), kmin(T(2), r));
+    } else {
+        (void) r;
+        return T(1);
+    }
+}
+
+// acoustic.cpp:10-19: linearised acoustic Riemann solver at the face between
+// the left cell (…_im) and the right cell (…_i).
+template <class T>
+ARMON_HD void godunov(T rho_i, T rho_im, T c_i, T c_im, T u_i, T u_im, T p_i, T p_im, T& ustar, T& pstar)
+{
+    T rc_l = rho_im * c_im;
+    T rc_r = rho_i * c_i;
+    ustar = (rc_l * u_im + rc_r * u_i + (p_im - p_i)) / (rc_l + rc_r);
+    pstar = (rc_r * p_im + rc_l * p_i + rc_l * rc_r * (u_im - u_i)) / (rc_l + rc_r);
+}
+
+// acoustic.cpp:65-86: GAD second-order correction at face i (between cells i-1 and i),
+// from the first-order solutions at faces i-1 (im), i and i+1 (ip).
+// Cell values: *_m = cell i-1, *_c = cell i.
+template <int L, class T>
+ARMON_HD void gad_correct(T dt, T dx,
+                          T rho_m, T rho_c, T c_m, T c_c, T u_m, T u_c, T p_m, T p_c,
+                          T ustar_im, T pstar_im, T ustar_i, T pstar_i, T ustar_ip, T pstar_ip,
+                          T& ustar, T& pstar)
+{
+    T r_um = (ustar_ip - u_c) / (ustar_i - u_m + T(1e-6));
+    T r_pm = (pstar_ip - p_c) / (pstar_i - p_m + T(1e-6));
+    T r_up = (u_m - ustar_im) / (u_c - ustar_i + T(1e-6));
+    T r_pp = (p_m - pstar_im) / (p_c - pstar_i + T(1e-6));
+
+    r_um = limiter<L>(r_um);
+    r_pm = limiter<L>(r_pm);
+    r_up = limiter<L>(r_up);
+    r_pp = limiter<L>(r_pp);
+
+    T dm_l = rho_m * dx;
+    T dm_r = rho_c * dx;
+    T Dm = (dm_l + dm_r) / 2;
+
+    T rc_l = rho_m * c_m;
+    T rc_r = rho_c * c_c;
+    T theta = T(0.5) * (1 - (rc_l + rc_r) / 2 * (dt / Dm));
+
+    ustar = ustar_i + theta * (r_up * (u_c - ustar_i) - r_um * (ustar_i - u_m));
+    pstar = pstar_i + theta * (r_pp * (p_c - pstar_i) - r_pm * (pstar_i - p_m));
+}
+
+// fluid_eos.cpp:16-21
+template <class T>
+ARMON_HD void perfect_gas(T gamma, T rho, T E, T u, T v, T& p, T& c, T& g)
+{
+    T e = E - T(0.5) * (u * u + v * v);
+    p = (gamma - 1) * rho * e;
+    c = sqrt_t<T>(gamma * p / rho);
+    g = (1 + gamma) / 2;
+}
+
+// fluid_eos.cpp:26-62. Computed in double: `e - eps_k0` and `p - pk0` cancel
+// almost completely near the reference density, which FP32 cannot resolve.
+ARMON_HD void bizarrium(double rho, double E, double u, double v, double& p_out, double& c_out, double& g_out)
+{
+    const double rho0 = 1e4, K0 = 1e11, Cv0 = 1e3, T0 = 300, eps0 = 0;
+    const double G0 = 1.5, s = 1.5, q = -42080895. / 14941154., r = 727668333. / 149411540.;
+
+    double x = rho / rho0 - 1;
+    double g = G0 * (1 - rho0 / rho);
+
+    double f0 = (1 + (s / 3 - 2) * x + q * (x * x) + r * (x * x * x)) / (1 - s * x);
+    double f1 = (s / 3 - 2 + 2 * q * x + 3 * r * (x * x) + s * f0) / (1 - s * x);
+    double f2 = (2 * q + 6 * r * x + 2 * s * f1) / (1 - s * x);
+    double f3 = (6 * r + 3 * s * f2) / (1 - s * x);
+
+    double eps_k0 = eps0 - Cv0 * T0 * (1 + g) + 0.5 * (K0 / rho0) * (x * x) * f0;
+    double pk0 = -Cv0 * T0 * G0 * rho0 + (0.5 * K0 * x * (1 + x) * (1 + x) * (2 * f0 + x * f1));
+    double pk0prime = -0.5 * K0 * cr_pow3(1 + x)
+            * rho0 * (2 * (1 + 3 * x) * f0 + 2 * x * (2 + 3 * x) * f1 + (x * x) * (1 + x) * f2);
+    double xp = x * (1 + x);
+    double pk0second = 0.5 * K0 * cr_pow4(1 + x) * (rho0 * rho0)
+            * (12 * (1 + 2 * x) * f0 + 6 * (1 + 6 * x + 6 * (x * x)) * f1 + 6 * x * (1 + x) * (1 + 2 * x) * f2
+               + (xp * xp) * f3);
+
+    double e = E - 0.5 * (u * u + v * v);
+    double p = pk0 + G0 * rho0 * (e - eps_k0);
+    double c = __builtin_sqrt(G0 * rho0 * (p - pk0) - pk0prime) / rho;
+    double G0rho0 = G0 * rho0;
+    g_out = 0.5 / (cr_pow3(rho) * (c * c)) * (pk0second + (G0rho0 * G0rho0) * (p - pk0));
+    p_out = p;
+    c_out = c;
+}
+
+// euler_projection.cpp:17-20: Lagrangian update of a cell from the face fluxes at i and i+1.
+template <class T>
+ARMON_HD void cell_update(T dx, T dt, T us_i, T us_ip, T ps_i, T ps_ip, T& rho, T& u, T& E)
+{
+    T dm = rho * dx;
+    rho = dm / (dx + dt * (us_ip - us_i));
+    u += dt / dm * (ps_i - ps_ip);
+    E += dt / dm * (ps_i * us_i - ps_ip * us_ip);
+}
+
+// euler_projection.cpp:73-79
+template <class T>
+ARMON_HD T slope_minmod(T u_im, T u_i, T u_ip, T r_m, T r_p)
+{
+    T D_u_p = r_p * (u_ip - u_i);
+    T D_u_m = r_m * (u_i - u_im);
+    T s = copysign_t<T>(T(1), D_u_p);
+    return s * kmax(T(0), kmin(s * D_u_p, s * D_u_m));
+}
+
+// euler_projection.cpp:62-69: donor-cell flux, given the donor cell state.
+template <class T>
+ARMON_HD void advection_flux_1(T disp, T rho, T u, T v, T E, T& f_rho, T& f_urho, T& f_vrho, T& f_Erho)
+{
+    f_rho = disp * (rho);
+    f_urho = disp * (rho * u);
+    f_vrho = disp * (rho * v);
+    f_Erho = disp * (rho * E);
+}
+
+// euler_projection.cpp:91-124: second-order flux at a face. The caller has applied the
+// donor shift: cell values *_m, *_c, *_p are cells d-1, d, d+1 around the donor cell d, and
+// us_m, us_c, us_p, us_pp are the face velocities at d-1, d, d+1, d+2 (face k sits between
+// cells k-1 and k). `Dx` is the signed upwind length chosen by the caller.
+template <class T>
+ARMON_HD void advection_flux_2(T dx, T dt, T disp, T Dx,
+                               T us_m, T us_c, T us_p, T us_pp,
+                               T rho_m, T rho_c, T rho_p, T u_m, T u_c, T u_p,
+                               T v_m, T v_c, T v_p, T E_m, T E_c, T E_p,
+                               T& f_rho, T& f_urho, T& f_vrho, T& f_Erho)
+{
+    T Dx_lm = dx + dt * (us_c - us_m);
+    T Dx_l = dx + dt * (us_p - us_c);
+    T Dx_lp = dx + dt * (us_pp - us_p);
+
+    T r_m = (2 * Dx_l) / (Dx_l + Dx_lm);
+    T r_p = (2 * Dx_l) / (Dx_l + Dx_lp);
+
+    T slope_r = slope_minmod(rho_m, rho_c, rho_p, r_m, r_p);
+    T slope_ur = slope_minmod(rho_m * u_m, rho_c * u_c, rho_p * u_p, r_m, r_p);
+    T slope_vr = slope_minmod(rho_m * v_m, rho_c * v_c, rho_p * v_p, r_m, r_p);
+    T slope_Er = slope_minmod(rho_m * E_m, rho_c * E_c, rho_p * E_p, r_m, r_p);
+
+    T length_factor = Dx / (2 * Dx_l);
+    f_rho = disp * (rho_c - slope_r * length_factor);
+    f_urho = disp * (rho_c * u_c - slope_ur * length_factor);
+    f_vrho = disp * (rho_c * v_c - slope_vr * length_factor);
+    f_Erho = disp * (rho_c * E_c - slope_Er * length_factor);
+}
+
+// euler_projection.cpp:42-52: remap of cell i from the advection fluxes at faces i, i+1.
+// `inv_dx` is used instead of a division only when dx is a power of two, where
+// x / dx == x * (1/dx) holds exactly; pass DX_POW2 = false otherwise.
+template <bool DX_POW2, class T>
+ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
+                         T f_rho_i, T f_rho_ip, T f_urho_i, T f_urho_ip,
+                         T f_vrho_i, T f_vrho_ip, T f_Erho_i, T f_Erho_ip,
+                         T& rho, T& u, T& v, T& E)
+{
+    T dX = dx + dt * (us_ip - us_i);
+    T n_rho, n_urho, n_vrho, n_Erho;
+    if constexpr (DX_POW2) {
+        n_rho = (dX * rho - (f_rho_ip - f_rho_i)) * inv_dx;
+        n_urho = (dX * rho * u - (f_urho_ip - f_urho_i)) * inv_dx;
+        n_vrho = (dX * rho * v - (f_vrho_ip - f_vrho_i)) * inv_dx;
+        n_Erho = (dX * rho * E - (f_Erho_ip - f_Erho_i)) * inv_dx;
+    } else {
+        (void) inv_dx;
+        n_rho = (dX * rho - (f_rho_ip - f_rho_i)) / dx;
+        n_urho = (dX * rho * u - (f_urho_ip - f_urho_i)) / dx;
+        n_vrho = (dX * rho * v - (f_vrho_ip - f_vrho_i)) / dx;
+        n_Erho = (dX * rho * E - (f_Erho_ip - f_Erho_i)) / dx;
+    }
+    rho = n_rho;
+    u = n_urho / n_rho;
+    v = n_vrho / n_rho;
+    E = n_Erho / n_rho;
+}
+
+// reductions.cpp:18-20: CFL bound of one cell (mask 1 for real cells).
+template <class T>
+ARMON_HD T cfl_cell(T dx, T dy, T u, T v, T c, T mask)
+{
+    T max_cx = kmax(abs_t<T>(u + c), abs_t<T>(u - c)) * mask;
+    T max_cy = kmax(abs_t<T>(v + c), abs_t<T>(v - c)) * mask;
+    return kmin(dx / max_cx, dy / max_cy);
+}
+
+}  // namespace armon

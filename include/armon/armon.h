@@ -1,0 +1,188 @@
+/*
+ * Armon MI355X library — extensions beside the drop-in kernel C-ABI (kernels.h).
+ *
+ *  1. Plumbing: library stream, device memory and copies (the caller owns memory
+ *     passed to the kernels; these helpers exist so that C/ctypes callers need no
+ *     other GPU runtime binding).
+ *  2. dt_CFL_device: the CFL reduction with its result left in device memory.
+ *  3. The host driver: Params, iteration domains and the time loop of the
+ *     reference (src/parameters.cpp, src/ranges.cpp, src/armon_2D.cpp), with two
+ *     engines:
+ *       ARMON_ENGINE_STAGED  one launch per stage through the drop-in entry points,
+ *                            host-side dt (the reference's schedule);
+ *       ARMON_ENGINE_FUSED   one fused Lagrange+remap sweep kernel per axis, the
+ *                            CFL reduction folded into the last sweep of a cycle,
+ *                            dt/time/cycle kept on the device, cycles replayed from
+ *                            a captured hipGraph.
+ *     The fused engine runs in FP64 (bit-compatible with the staged engine) or in
+ *     FP32-mixed precision (FP32 storage and flux arithmetic; FP64 EOS, dt and
+ *     conservation sums).
+ *
+ * All functions return 0 on success unless stated otherwise; on failure the error
+ * also goes through raise_exception_handler and armon_last_error().
+ */
+#ifndef ARMON_HIP_ARMON_H
+#define ARMON_HIP_ARMON_H
+
+#include <stddef.h>
+
+#include "armon/kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- plumbing ---------------------------------------------------------- */
+
+/* Stream used by every launch of the library (a hipStream_t; NULL = default stream). */
+ARMON_EXPORT int armon_set_stream(void* hip_stream);
+ARMON_EXPORT void* armon_get_stream(void);
+
+ARMON_EXPORT int armon_device_count(void);
+ARMON_EXPORT void* armon_malloc(size_t bytes);            /* device memory, 256-byte aligned; NULL on failure */
+ARMON_EXPORT int armon_free(void* device_ptr);
+ARMON_EXPORT int armon_memcpy_htod(void* dst, const void* src, size_t bytes);  /* synchronous */
+ARMON_EXPORT int armon_memcpy_dtoh(void* dst, const void* src, size_t bytes);  /* synchronous */
+ARMON_EXPORT int armon_memcpy_dtod(void* dst, const void* src, size_t bytes);  /* async on the library stream */
+ARMON_EXPORT int armon_synchronize(void);                 /* waits for the library stream */
+ARMON_EXPORT const char* armon_last_error(void);          /* "" when no error was recorded */
+ARMON_EXPORT void armon_clear_error(void);
+ARMON_EXPORT const char* armon_version(void);
+
+/* ---- device-resident CFL reduction -------------------------------------- */
+
+/* Same reduction as dt_CFL (reductions.cpp:8-24) but asynchronous: the minimum is
+ * written to *dt_device (a device address) on the library stream. */
+ARMON_EXPORT void dt_CFL_device(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D) inner_range, flt_t dx, flt_t dy,
+                                ARMON_CREF(view) umat, ARMON_CREF(view) vmat, ARMON_CREF(view) cmat,
+                                ARMON_CREF(mask_view) domain_mask, double* dt_device);
+
+/* ---- parameters (src/parameters.h:17-108) -------------------------------- */
+
+enum { ARMON_SCHEME_GODUNOV = 0, ARMON_SCHEME_GAD = 1 };
+enum { ARMON_RIEMANN_ACOUSTIC = 0 };
+enum { ARMON_PROJ_NONE = 0, ARMON_PROJ_EULER = 1, ARMON_PROJ_EULER_2ND = 2 };
+enum { ARMON_SPLIT_SEQUENTIAL = 0, ARMON_SPLIT_SEQUENTIAL_SYM = 1, ARMON_SPLIT_STRANG = 2,
+       ARMON_SPLIT_X_ONLY = 3, ARMON_SPLIT_Y_ONLY = 4 };
+enum { ARMON_AXIS_X = 0, ARMON_AXIS_Y = 1 };
+enum { ARMON_SIDE_LEFT = 0, ARMON_SIDE_RIGHT = 1, ARMON_SIDE_TOP = 2, ARMON_SIDE_BOTTOM = 3 };
+enum { ARMON_ENGINE_STAGED = 0, ARMON_ENGINE_FUSED = 1 };
+enum { ARMON_PREC_FP64 = 0, ARMON_PREC_FP32_MIXED = 1 };
+
+typedef struct ArmonParams {
+    int test, scheme, riemann, projection, limiter, current_axis, axis_splitting;
+    int nb_ghosts, nx, ny;
+    double dx, cfl, Dt;
+    int stencil_width;              /* -1: min_ghosts() */
+    int cst_dt, dt_on_even_cycles;
+    double domain_size[2];          /* <= 0: test default */
+    double domain_origin[2];        /* NaN: test default */
+    /* indexing (set by armon_params_init) */
+    int row_length, col_length, nb_cells, ideb, ifin, index_start, idx_row, idx_col;
+    int s;
+    /* solver state */
+    int cycle;
+    double cycle_dt, current_cycle_dt, next_cycle_dt, time;
+    /* bounds */
+    int max_cycles;
+    double max_time;
+    /* output */
+    int verbose;
+    int write_output, write_ghosts, write_throughput, output_precision;
+    const char* output_file;
+    /* execution options of this implementation */
+    int engine;                     /* ARMON_ENGINE_* */
+    int precision;                  /* ARMON_PREC_* (fused engine only) */
+    int graph_chunk;                /* cycles per captured graph launch (fused engine); 0: no graph */
+} ArmonParams;
+
+ARMON_EXPORT void armon_params_default(ArmonParams* p);              /* parameters.h:34-97 defaults */
+ARMON_EXPORT int armon_params_init(ArmonParams* p);                  /* parameters.cpp:21-67 */
+ARMON_EXPORT int armon_params_check(const ArmonParams* p);           /* parameters.cpp:70-109: 1 if valid */
+ARMON_EXPORT int armon_min_ghosts(const ArmonParams* p);             /* parameters.cpp:11-18 */
+ARMON_EXPORT void armon_update_axis(ArmonParams* p, int axis);       /* parameters.cpp:195-210 */
+ARMON_EXPORT int armon_split_axes(const ArmonParams* p, int axes[3], double factors[3]); /* parameters.cpp:212-255 */
+ARMON_EXPORT void armon_test_boundary_condition(int test, int side, double* u_factor, double* v_factor);
+ARMON_EXPORT double armon_test_option(const ArmonParams* p);         /* test_cases.h:32, 118-120 */
+
+/* ---- iteration domains (src/kernels/indexing.h:93-116, src/ranges.cpp) ---- */
+
+typedef struct DomainRange {
+    int64_t col_start, col_step, col_end;   /* *_end inclusive */
+    int64_t row_start, row_step, row_end;
+} DomainRange;
+
+ARMON_EXPORT uint64_t armon_domain_col_length(const DomainRange* d);
+ARMON_EXPORT uint64_t armon_domain_row_length(const DomainRange* d);
+ARMON_EXPORT uint64_t armon_domain_length(const DomainRange* d);
+ARMON_EXPORT int64_t armon_domain_begin(const DomainRange* d);
+ARMON_EXPORT int64_t armon_domain_end(const DomainRange* d);
+ARMON_EXPORT void armon_domain_iter1D(const DomainRange* d, Range* r, InnerRange1D* ir);
+ARMON_EXPORT void armon_domain_direct_iter1D(const DomainRange* d, Range* r, InnerRange1D* ir);
+ARMON_EXPORT void armon_domain_iter2D(const DomainRange* d, Range* r, InnerRange2D* ir);
+
+ARMON_EXPORT int64_t armon_index_1D(const ArmonParams* p, int i, int j);
+ARMON_EXPORT DomainRange armon_real_domain(const ArmonParams* p);
+ARMON_EXPORT DomainRange armon_domain_fluxes(const ArmonParams* p);
+ARMON_EXPORT DomainRange armon_domain_cell_update(const ArmonParams* p);
+ARMON_EXPORT DomainRange armon_domain_advection(const ArmonParams* p);
+ARMON_EXPORT DomainRange armon_complete_domain(const ArmonParams* p);
+ARMON_EXPORT DomainRange armon_boundary_conditions_domain(const ArmonParams* p, int side, int* disp);
+
+/* ---- solver (src/armon_2D.cpp) ------------------------------------------ */
+
+/* Field identifiers, in the order of data.h:103-106 plus the mask. */
+enum { ARMON_FIELD_X = 0, ARMON_FIELD_Y, ARMON_FIELD_RHO, ARMON_FIELD_UMAT, ARMON_FIELD_VMAT,
+       ARMON_FIELD_EMAT, ARMON_FIELD_PMAT, ARMON_FIELD_CMAT, ARMON_FIELD_GMAT, ARMON_FIELD_USTAR,
+       ARMON_FIELD_PSTAR, ARMON_FIELD_WORK1, ARMON_FIELD_WORK2, ARMON_FIELD_WORK3, ARMON_FIELD_WORK4,
+       ARMON_FIELD_DOMAIN_MASK, ARMON_FIELD_COUNT };
+
+typedef struct ArmonResult {
+    int status;          /* 0 ok, 1 invalid time step (the reference exits), 2 other error */
+    int cycles;
+    double time;
+    double current_dt;   /* dt of the last cycle */
+    double final_dt;     /* next_cycle_dt, as printed by the reference */
+    double loop_time;    /* seconds, EOS_init + loop + final fence (armon_2D.cpp:334-373) */
+    double grind_time;   /* microseconds per cell per cycle */
+    double initial_mass, initial_energy, final_mass, final_energy;
+} ArmonResult;
+
+typedef struct ArmonKernelTimes {
+    int launches[4];      /* [0] X sweeps, [1] Y sweeps, [2] time-step kernels, [3] other */
+    double total_ms[4];   /* summed HIP-event durations per kernel class */
+} ArmonKernelTimes;
+
+typedef struct ArmonSolver ArmonSolver;
+
+/* Allocates the device data for `p` (which must have been through armon_params_init). */
+ARMON_EXPORT ArmonSolver* armon_solver_create(const ArmonParams* p);
+ARMON_EXPORT void armon_solver_destroy(ArmonSolver* s);
+ARMON_EXPORT int armon_solver_get_params(const ArmonSolver* s, ArmonParams* out);
+/* armon_2D.cpp:112-127 */
+ARMON_EXPORT int armon_solver_init_test(ArmonSolver* s, bool debug_indexes);
+/* armon_2D.cpp:327-386: EOS_init, then cycles while time < max_time && cycle < max_cycles. */
+ARMON_EXPORT int armon_solver_time_loop(ArmonSolver* s, ArmonResult* out);
+/* Enqueues up to n more cycles (still bounded by max_time/max_cycles on the device)
+ * without synchronising; armon_solver_time_loop or armon_solver_begin must come first. */
+ARMON_EXPORT int armon_solver_begin(ArmonSolver* s);
+ARMON_EXPORT int armon_solver_advance(ArmonSolver* s, int n);
+ARMON_EXPORT int armon_solver_finish(ArmonSolver* s, ArmonResult* out);
+/* Runs n cycles launch by launch, timing every kernel with HIP events on the solver stream. */
+ARMON_EXPORT int armon_solver_kernel_times(ArmonSolver* s, int n, ArmonKernelTimes* out);
+/* Copies a whole field (nb_cells values) to/from the host as doubles. For the fused
+ * engine, fields are the current state; PMAT/CMAT hold the EOS of the input of the
+ * last sweep, exactly as in the reference. */
+ARMON_EXPORT int armon_solver_read_field(ArmonSolver* s, int field, double* host, int64_t n);
+ARMON_EXPORT int armon_solver_write_field(ArmonSolver* s, int field, const double* host, int64_t n);
+ARMON_EXPORT int armon_solver_conservation(ArmonSolver* s, double* mass, double* energy);
+ARMON_EXPORT void* armon_solver_stream(ArmonSolver* s);
+
+/* armon_2D.cpp:389-431: allocate, init, loop, optional output. */
+ARMON_EXPORT int armon_run(ArmonParams* p, ArmonResult* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARMON_HIP_ARMON_H */

@@ -1,0 +1,38 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    config.addinivalue_line("markers", "slow: long-running parity case")
+
+
+@pytest.fixture(scope="session")
+def armon():
+    import __graft_entry__ as g
+
+    mod = g.import_package()
+    mod.load()
+    mod.errors()
+    return mod
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import pyoracle as po
+
+    return po.load(strict=True)
+
+
+@pytest.fixture(scope="session")
+def po():
+    import pyoracle
+
+    return pyoracle

@@ -1,0 +1,203 @@
+"""Whole-run parity of the solver (staged drop-in engine and fused engine) against the oracle.
+
+Both engines compute in FP64 without FMA contraction, in the reference's operation order, so
+the final state, the cycle count and the final time step must match the oracle bit for bit.
+The reference's own whole-run known answers (cycle counts of tests/armon_2D_tests.cpp:238-242,
+conservation to 1e-12) are checked on the GPU as well.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("rho", "umat", "vmat", "Emat", "pmat", "cmat")
+ENGINES = {"staged": 0, "fused": 1}
+
+
+def oracle_run(po, oracle, test, **cfg):
+    p = po.make_params(oracle, test, **cfg)
+    st, d, cycles, final_dt, _ = po.run(oracle, p)
+    return st, d, cycles, final_dt
+
+
+def product_run(armon, test, engine, init=None, **cfg):
+    p = armon.make_params(test, engine=engine, **cfg)
+    r, s = armon.run(p, init)
+    return r, s
+
+
+def report(name, a, b):
+    d = np.abs(a - b)
+    rel = d / np.maximum(np.abs(b), 1e-300)
+    return f"{name}: {int(np.count_nonzero(a != b))} differ, max abs {d.max():.3e}, max rel {rel.max():.3e}"
+
+
+def assert_same_state(s, od, fields=STATE):
+    mask = od.array("domain_mask") == 1
+    msgs = []
+    for f in fields:
+        a, b = s.read(f)[mask], od.array(f)[mask]
+        if not np.array_equal(a.view(np.uint64), b.view(np.uint64)):
+            msgs.append(report(f, a, b))
+    assert not msgs, "; ".join(msgs)
+
+
+def check(armon, po, oracle, test, engine, fields=STATE, **cfg):
+    st, od, cycles, final_dt = oracle_run(po, oracle, test, **cfg)
+    assert st == 0
+    print(f"case {test} engine={engine} {cfg}", flush=True)
+    r, s = product_run(armon, test, engine, **cfg)
+    assert r.status == 0, armon.errors()
+    assert r.cycles == cycles
+    assert r.final_dt == final_dt, (r.final_dt, final_dt)
+    assert_same_state(s, od, fields)
+    s.close()
+    return cycles
+
+
+REF = dict(scheme=1, projection=2, limiter=1, axis_splitting=0, nb_ghosts=5, nx=100, ny=100, max_cycles=1000,
+           verbose=5)
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+@pytest.mark.parametrize("test,cycles", [("Sod", 45), ("Sod_y", 45), ("Sod_circ", 43), ("Bizarrium", 76),
+                                         ("Sedov", 568)])
+def test_reference_cycle_counts(armon, po, oracle, engine, test, cycles):
+    """tests/armon_2D_tests.cpp:238-242 with the reference parameters (reference.cpp:33-64)."""
+    assert check(armon, po, oracle, test, ENGINES[engine], **REF) == cycles
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_sod_first_order(armon, po, oracle, engine):
+    """Sod 100x100, Godunov + first-order remap, CFL 0.6, to t = 0.2, default 3 ghosts and a
+    stencil width of 1: the second ghost layer is read but never refilled (stale values)."""
+    cfg = dict(scheme=0, projection=1, cfl=0.6, max_time=0.2, nx=100, ny=100, max_cycles=100000, verbose=5)
+    check(armon, po, oracle, "Sod", ENGINES[engine], **cfg)
+
+
+@pytest.mark.parametrize("scheme,proj,limiter", [(0, 1, 1), (0, 2, 1), (1, 1, 1), (1, 2, 0), (1, 2, 2)])
+def test_fused_schemes(armon, po, oracle, scheme, proj, limiter):
+    cfg = dict(scheme=scheme, projection=proj, limiter=limiter, nb_ghosts=4, nx=64, ny=48, max_cycles=40,
+               verbose=5)
+    check(armon, po, oracle, "Sedov", 1, **cfg)
+
+
+@pytest.mark.parametrize("split", [1, 2, 3, 4])
+@pytest.mark.parametrize("test", ["Sedov", "Sod_circ"])
+def test_fused_splittings(armon, po, oracle, split, test):
+    cfg = dict(scheme=1, projection=2, limiter=1, axis_splitting=split, nb_ghosts=4, nx=50, ny=70, max_cycles=30,
+               verbose=5)
+    check(armon, po, oracle, test, 1, **cfg)
+
+
+@pytest.mark.parametrize("limiter", [1, 2])
+def test_fused_bizarrium(armon, po, oracle, limiter):
+    cfg = dict(scheme=1, projection=2, limiter=limiter, nb_ghosts=4, nx=96, ny=64, max_cycles=60, verbose=5)
+    check(armon, po, oracle, "Bizarrium", 1, **cfg)
+
+
+def test_fused_stencil_and_ghosts(armon, po, oracle):
+    """More ghost layers than the stencil reads, and a stencil wider than the minimum."""
+    cfg = dict(scheme=1, projection=1, nb_ghosts=6, stencil_width=5, nx=40, ny=36, max_cycles=30, verbose=5)
+    check(armon, po, oracle, "Sod_circ", 1, **cfg)
+    cfg = dict(scheme=0, projection=2, nb_ghosts=3, nx=40, ny=36, max_cycles=30, verbose=5)   # stale 3rd layer
+    check(armon, po, oracle, "Sedov", 1, **cfg)
+
+
+@pytest.mark.parametrize("chunk", [0, 2, 6])
+def test_fused_graph_chunks(armon, po, oracle, chunk):
+    cfg = dict(scheme=1, projection=2, nb_ghosts=4, nx=64, ny=64, max_cycles=23, verbose=5)
+    st, od, cycles, final_dt = oracle_run(po, oracle, "Sedov", **cfg)
+    r, s = product_run(armon, "Sedov", 1, graph_chunk=chunk, **cfg)
+    assert r.status == 0 and r.cycles == cycles == 23 and r.final_dt == final_dt
+    assert_same_state(s, od)
+
+
+def test_fused_time_bound(armon, po, oracle):
+    """The loop stops on max_time with the same cycle count (device-side condition)."""
+    cfg = dict(scheme=1, projection=2, nb_ghosts=4, nx=64, ny=64, max_cycles=100000, max_time=0.05, verbose=5)
+    check(armon, po, oracle, "Sedov", 1, **cfg)
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_constant_dt(armon, po, oracle, engine):
+    cfg = dict(scheme=1, projection=2, nb_ghosts=4, nx=48, ny=48, max_cycles=20, cst_dt=1, Dt=1e-3, verbose=5)
+    check(armon, po, oracle, "Sod_circ", ENGINES[engine], **cfg)
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_dt_on_even_cycles(armon, po, oracle, engine):
+    cfg = dict(scheme=1, projection=2, nb_ghosts=4, nx=48, ny=48, max_cycles=20, dt_on_even_cycles=1, verbose=5)
+    check(armon, po, oracle, "Sedov", ENGINES[engine], **cfg)
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_invalid_time_step(armon, po, oracle, engine):
+    """Zero sound speed and velocity everywhere: dt_CFL returns DBL_MAX and CFL 2 overflows it
+    to +inf, an invalid time step (armon_2D.cpp:238-242) at cycle 0."""
+    cfg = dict(scheme=1, projection=2, nb_ghosts=4, nx=32, ny=32, max_cycles=5, cfl=2.0, verbose=5)
+    p = po.make_params(oracle, "Sod", **cfg)
+    d = po.Data(oracle, p.nb_cells)
+    oracle.o_init_test_params(C.byref(p), C.byref(d.c), False)
+    for f in ("Emat", "umat", "vmat"):
+        d.array(f)[:] = 0.0
+    lt, fdt, cyc = C.c_double(), C.c_double(), C.c_int()
+    assert oracle.o_time_loop(C.byref(p), C.byref(d.c), C.byref(lt), C.byref(fdt), C.byref(cyc)) == 1
+    q = armon.make_params("Sod", engine=ENGINES[engine], **cfg)
+    s = armon.Solver(q)
+    s.init_test()
+    z = np.zeros(q.nb_cells)
+    for f in ("Emat", "umat", "vmat"):
+        s.write(f, z)
+    r = s.time_loop()
+    errs = armon.errors()
+    assert r.status == 1
+    assert any("Invalid dt at cycle 0" in m for _, m in errs), errs
+
+
+def test_fused_conservation(armon):
+    """armon_2D_tests.cpp:246-275 on the fused engine."""
+    for test in ("Sod", "Sod_y", "Sod_circ", "Sedov"):
+        p = armon.reference_params(test, max_time=10000.0, max_cycles=400, engine=1)
+        s = armon.Solver(p)
+        s.init_test()
+        m0, e0 = C.c_double(), C.c_double()
+        lib = armon.load()
+        s.begin()
+        lib.armon_solver_conservation(s.h, C.byref(m0), C.byref(e0))
+        s.advance(400)
+        r = s.finish()
+        m1, e1 = C.c_double(), C.c_double()
+        lib.armon_solver_conservation(s.h, C.byref(m1), C.byref(e1))
+        assert r.status == 0 and r.cycles == 400
+        assert abs(m1.value - m0.value) <= 1e-12 * abs(m0.value), test
+        assert abs(e1.value - e0.value) <= 1e-12 * abs(e0.value), test
+        s.close()
+
+
+def test_sedov_1024_fused(armon, po, oracle):
+    """Sedov 1024x1024 (GAD + minmod, second-order remap, CFL 0.7) for 60 cycles, bit for bit."""
+    cfg = dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=1024, ny=1024, max_cycles=60, verbose=5)
+    check(armon, po, oracle, "Sedov", 1, **cfg)
+
+
+@pytest.mark.parametrize("test", ["Sedov", "Sod_circ"])
+def test_fp32_mixed_accuracy(armon, test):
+    """FP32 storage and flux arithmetic against the FP64 engine after 100 cycles: relative L1
+    error of the density <= 1e-5 (and the other fields reported)."""
+    cfg = dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=256, ny=256, max_cycles=100, verbose=5)
+    r64, s64 = product_run(armon, test, 1, **cfg)
+    r32, s32 = product_run(armon, test, 1, precision=1, **cfg)
+    assert r64.status == 0 and r32.status == 0 and r32.cycles == r64.cycles == 100
+    p = s64.params()
+    mask = s64.read("domain_mask") == 1
+    out = {}
+    for f in ("rho", "umat", "vmat", "Emat", "pmat"):
+        a, b = s32.read(f)[mask], s64.read(f)[mask]
+        out[f] = (np.sum(np.abs(a - b)) / np.sum(np.abs(b)), np.max(np.abs(a - b)))
+    print(test, {k: f"L1={v[0]:.2e} Linf={v[1]:.2e}" for k, v in out.items()}, file=sys.stderr)
+    assert out["rho"][0] <= 1e-5
+    assert p.nx == 256
