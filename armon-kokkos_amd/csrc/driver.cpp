@@ -142,6 +142,7 @@ struct ArmonSolver {
     double initial_mass = 0, initial_energy = 0;
     std::chrono::steady_clock::time_point t_begin;
     int error_count_at_begin = 0;
+    bool dt_error = false;           // an invalid time step stopped the loop (armon_2D.cpp:238-242)
 
     ~ArmonSolver();
 };
@@ -286,6 +287,7 @@ bool staged_time_step(ArmonSolver* s)
             char buf[128];
             snprintf(buf, sizeof buf, "Invalid dt at cycle %d: %f", p.cycle, local_dt);
             g_error_count++;
+            s->dt_error = true;
             report("time_step", Error("std::runtime_error", buf));
             return false;
         }
@@ -989,6 +991,7 @@ ARMON_EXPORT int armon_solver_begin(ArmonSolver* s)
         StreamScope scope(s->stream);
         ArmonParams& p = s->p;
         s->error_count_at_begin = g_error_count;
+        s->dt_error = false;
         p.cycle = 0;
         p.time = 0.;
         p.current_cycle_dt = 0.;
@@ -1067,6 +1070,7 @@ ARMON_EXPORT int armon_solver_finish(ArmonSolver* s, ArmonResult* out)
             }
         } else {
             hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+            if (s->dt_error) status = 1;
         }
         const double loop_time =
                 std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_begin).count();

@@ -1,0 +1,167 @@
+"""Benchmark: cell-updates/s of the Armon Lagrange + remap step on one MI355X.
+
+Workload: Sedov blast, 8192 x 8192 real cells, 4 ghost layers, GAD + minmod fluxes,
+second-order remap, Sequential X/Y splitting, CFL 0.7, perfect gas gamma = 1.4, reference
+initial state (blast in the central 2x2 cells of [-1, 1]^2). One step = one full cycle (the
+X sweep, the Y sweep and the CFL time step), run by the fused engine with the state resident
+in HBM and cycles replayed from a captured hipGraph.
+
+Printed on one JSON line: the FP64 rate as `value`, the FP32-mixed rate beside it, the HBM
+roofline of the dominant kernel (timed with HIP events on the solver stream), and the CPU
+baseline (the oracle's fast-math build on this host's cores, on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
+WORKLOAD = dict(scheme=1, projection=2, limiter=1, axis_splitting=0, nb_ghosts=4, cfl=0.7)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles):
+    p = armon.make_params("Sedov", nx=nx, ny=ny, max_cycles=10**9, max_time=1e9, verbose=5, engine=1,
+                          precision=precision, graph_chunk=16, **WORKLOAD)
+    s = armon.Solver(p)
+    s.init_test()
+    lib = armon.load()
+    stream = lib.armon_solver_stream(s.h)
+
+    def sync():
+        lib.armon_set_stream(stream)
+        lib.armon_synchronize()
+        lib.armon_set_stream(None)
+
+    s.begin()
+    s.advance(warmup)
+    sync()
+    # timed region: exactly `steps` cycles, synchronised on both sides
+    t0 = time.perf_counter()
+    s.advance(steps)
+    sync()
+    elapsed = time.perf_counter() - t0
+    r = armon.ArmonResult()
+    st = lib.armon_solver_finish(s.h, C.byref(r))
+    if st != 0 or r.cycles != warmup + steps:
+        raise RuntimeError(f"run failed: status {st}, cycles {r.cycles}, {armon.errors()}")
+    rho = s.read("rho")
+    kt = s.kernel_times(profile_cycles)
+    s.close()
+    return elapsed, kt, rho
+
+
+def cpu_baseline(nx, ny, cycles):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    lib = po.load(strict=False)
+    threads = lib.o_num_threads()
+    p = po.make_params(lib, "Sedov", nx=nx, ny=ny, max_cycles=cycles, verbose=5, **WORKLOAD)
+    st, d, ncyc, _, loop_time = po.run(lib, p)
+    del d
+    if st != 0 or ncyc != cycles:
+        raise RuntimeError("CPU baseline run failed")
+    return nx * ny * cycles / loop_time, threads, loop_time
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=8192)
+    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as g
+
+    armon = g.import_package()
+    armon.load()
+    n = args.cells
+    cells = n * n
+
+    log(f"[bench] FP64 fused, Sedov {n}x{n}, warmup {args.warmup}, steps {args.steps}")
+    el64, kt64, rho64 = measure_gpu(armon, n, n, args.steps, args.warmup, 0, 10)
+    rate64 = cells * args.steps / el64
+    log(f"[bench] FP64: {rate64 / 1e9:.3f} G cell-updates/s, {el64 / args.steps * 1e3:.3f} ms/cycle")
+
+    # dominant kernel and its roofline (bytes per launch: read 4 + write 4 FP64 state words per cell)
+    names = ["sweep_x", "sweep_y", "time_step"]
+    dom = max(range(2), key=lambda k: kt64.total_ms[k])
+    avg_s = kt64.total_ms[dom] / kt64.launches[dom] / 1e3
+    bytes_per_launch = 64 * cells
+    achieved = bytes_per_launch / avg_s
+    sweep_avg = {names[k]: kt64.total_ms[k] / max(kt64.launches[k], 1) for k in range(3)}
+
+    fp32 = None
+    if not args.no_fp32:
+        log("[bench] FP32-mixed fused")
+        el32, kt32, rho32 = measure_gpu(armon, n, n, args.steps, args.warmup, 1, 10)
+        rate32 = cells * args.steps / el32
+        real = np.zeros_like(rho64, dtype=bool)
+        g_ = WORKLOAD["nb_ghosts"]
+        rl = n + 2 * g_
+        real.reshape(rl, rl)[g_:g_ + n, g_:g_ + n] = True
+        l1 = float(np.sum(np.abs(rho32[real] - rho64[real])) / np.sum(np.abs(rho64[real])))
+        linf = float(np.max(np.abs(rho32[real] - rho64[real])))
+        fp32 = {"value": rate32, "ms_per_step": el32 / args.steps * 1e3, "speedup_vs_fp64": rate32 / rate64,
+                "rho_rel_L1_vs_fp64": l1, "rho_Linf_vs_fp64": linf,
+                "sweep_ms": {names[k]: kt32.total_ms[k] / max(kt32.launches[k], 1) for k in range(3)}}
+        log(f"[bench] FP32-mixed: {rate32 / 1e9:.3f} G cell-updates/s, speedup {rate32 / rate64:.2f}, "
+            f"rho L1 {l1:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: oracle fast-math build, Sedov {n}x{n}, {args.cpu_cycles} cycles")
+        rate_cpu, threads, secs = cpu_baseline(n, n, args.cpu_cycles)
+        cpu = {"value": rate_cpu, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"Sedov {n}x{n}, {args.cpu_cycles} cycles (15 per-stage loops per cycle, "
+                         f"-O3 -ffast-math OpenMP build of the oracle), {secs:.1f} s"}
+        log(f"[bench] CPU: {rate_cpu / 1e6:.1f} M cell-updates/s on {threads} threads")
+
+    out = {
+        "metric": "cell-updates/s at 8192^2 Sedov (FP64 fused; FP32-mixed beside it)",
+        "value": rate64,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el64 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (reference Sedov initial state)",
+        "config": {"workload": f"Sedov {n}x{n}, 4 ghosts, GAD+minmod, Euler 2nd order remap, Sequential "
+                               f"splitting, CFL 0.7, fused engine, hipGraph of 16 cycles",
+                   "cells": cells, "engine": "fused"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": names[dom],
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
+                     "kernel_avg_ms": sweep_avg},
+        "fp32_mixed": fp32,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
