@@ -135,6 +135,7 @@ EXTENSION_SYMBOLS = {
     "armon_last_error": (C.c_char_p, []),
     "armon_clear_error": (None, []),
     "armon_version": (C.c_char_p, []),
+    "armon_copy_bandwidth": (C.c_double, [C.c_size_t, C.c_int]),
     "dt_CFL_device": (None, [_R, _I1, _d, _d, _V, _V, _V, _V, C.c_void_p]),
     "armon_params_default": (None, [_PP]),
     "armon_params_init": (C.c_int, [_PP]),

@@ -382,7 +382,7 @@ SweepArgs<T> sweep_args(ArmonSolver* s, int axis, double factor, bool reduce, T*
     a.row_length = p.row_length;
     a.index_start = p.index_start;
     a.stencil_width = p.stencil_width;
-    a.seg_len = y_segment_rows(p.nx, p.ny);
+    a.seg_len = 0;
     a.gamma = 1.4;
     a.dx = axis == ARMON_AXIS_X ? p.domain_size[0] / (double) p.nx : p.domain_size[1] / (double) p.ny;
     a.dx_pow2 = is_pow2(a.dx) ? 1 : 0;

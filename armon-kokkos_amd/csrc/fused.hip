@@ -291,6 +291,7 @@ __global__ void __launch_bounds__(kYBlock) k_sweep_y(SweepArgs<T> a, int ncolblk
 
     double local = DBL_MAX;
 
+#pragma unroll 4
     for (int n = y0 - R; n < y1 + R; ++n) {
         const Cell<T> cn = lane_ok ? load_cell<T, 1, EOS>(a, x, n, a.ny) : benign_cell<T>();
         pre[0] = pre[1];
@@ -475,6 +476,29 @@ unsigned grid_for(int64_t n, int block, int64_t max_blocks)
     return (unsigned) g;
 }
 
+// Blocks of `block` threads of `kernel` resident on the whole device at once.
+int resident_blocks(const void* kernel, int block)
+{
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+// Rows per marching run: the column blocks times the runs per column fill one resident round,
+// but a run is kept at least 16 halo widths long so that its warm-up stays small.
+int y_run_rows(int ny, int ncolblk, int resident, int halo)
+{
+    int nseg = resident > 0 ? resident / (ncolblk > 0 ? ncolblk : 1) : 1;
+    if (nseg < 1) nseg = 1;
+    int len = (ny + nseg - 1) / nseg;
+    const int min_len = 16 * halo;
+    if (len < min_len) len = min_len;
+    if (len > ny) len = ny;
+    return len > 0 ? len : 1;
+}
+
 void check_launch(const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -491,9 +515,16 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
         k_sweep_x<T, F, P, E><<<dim3((unsigned) (nseg * a.ny)), dim3(kTile), 0, s>>>(a, nseg);
         check_launch("sweep_x");
     } else {
+        // One resident round of blocks: each block marches down a contiguous run of rows of
+        // its 256 columns, so the warm-up is paid once per run and no partial second round
+        // of blocks leaves CUs idle.
+        static int resident = 0;
+        if (resident == 0) resident = resident_blocks((const void*) k_sweep_y<T, F, P, E>, kYBlock);
         const int ncolblk = (a.nx + kYBlock - 1) / kYBlock;
-        const int nseg = (a.ny + a.seg_len - 1) / a.seg_len;
-        k_sweep_y<T, F, P, E><<<dim3((unsigned) (ncolblk * nseg)), dim3(kYBlock), 0, s>>>(a, ncolblk);
+        SweepArgs<T> b = a;
+        b.seg_len = y_run_rows(a.ny, ncolblk, resident, R);
+        const int nseg = (a.ny + b.seg_len - 1) / b.seg_len;
+        k_sweep_y<T, F, P, E><<<dim3((unsigned) (ncolblk * nseg)), dim3(kYBlock), 0, s>>>(b, ncolblk);
         check_launch("sweep_y");
     }
 }
@@ -527,19 +558,6 @@ void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStr
 
 template void launch_sweep<double>(int, const SweepVariant&, const SweepArgs<double>&, hipStream_t);
 template void launch_sweep<float>(int, const SweepVariant&, const SweepArgs<float>&, hipStream_t);
-
-int y_segment_rows(int nx, int ny)
-{
-    // Aim at ~4096 waves of 64 columns in total, with segments of at least 32 rows so that
-    // the 2R-row warm-up of the march stays a small fraction of the work.
-    const int waves_per_row = (nx + 63) / 64;
-    int nseg = 4096 / (waves_per_row > 0 ? waves_per_row : 1);
-    if (nseg < 1) nseg = 1;
-    int len = (ny + nseg - 1) / nseg;
-    if (len < 32) len = 32;
-    if (len > ny) len = ny;
-    return len > 0 ? len : 1;
-}
 
 int x_tile_outputs(const SweepVariant& v) { return kTile - 2 * read_extent(v.flux, v.proj); }
 

@@ -33,7 +33,8 @@ struct SweepArgs {
     T* out_rho; T* out_u; T* out_v; T* out_E;
     const T* stale_p; const T* stale_c;   // pmat/cmat, read only for ghost layers beyond the stencil width
     DeviceState* st;
-    int nx, ny, row_length, index_start, stencil_width, seg_len;
+    int nx, ny, row_length, index_start, stencil_width;
+    int seg_len;                          // Y sweep: rows per marching run (set by the launcher)
     int dx_pow2, reduce;
     double gamma, dx, inv_dx, dt_factor;
     double lo_u, lo_v, hi_u, hi_v;        // BC factors: Left/Bottom (lo), Right/Top (hi)
@@ -53,8 +54,7 @@ inline int read_extent(int flux, int proj) { return 2 + (flux > 0 ? 1 : 0) + (pr
 template <class T>
 void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStream_t s);
 
-// Segment length of the Y march and the X tile width, chosen for the domain size.
-int y_segment_rows(int nx, int ny);
+// Output cells per X-sweep tile.
 int x_tile_outputs(const SweepVariant& v);
 
 void launch_time_step(DeviceState* st, const TimeStepConfig& c, hipStream_t s);

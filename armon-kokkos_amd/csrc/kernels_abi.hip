@@ -1042,6 +1042,59 @@ ARMON_KERNEL_TRY
 }
 ARMON_KERNEL_CATCH("conservation_vars")
 
+// ---- measurement -----------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+// Plain streaming copy (16-byte lanes, grid-stride): the achievable HBM bandwidth of this
+// device, against which the sweeps' traffic and rate are compared.
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ in, double2* __restrict__ out, int64_t n)
+{
+    for (int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t) gridDim.x * blockDim.x)
+        out[k] = in[k];
+}
+}  // namespace
+
+extern "C" {
+
+ARMON_EXPORT double armon_copy_bandwidth(size_t bytes, int iters)
+{
+    void* a = nullptr;
+    void* b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double gbps = -1.0;
+    try {
+        bytes = bytes / 16 * 16;
+        hip_check(hipMalloc(&a, bytes), "hipMalloc(copy)");
+        hip_check(hipMalloc(&b, bytes), "hipMalloc(copy)");
+        hip_check(hipMemsetAsync(a, 0, bytes, stream()), "hipMemset(copy)");
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const int64_t n = (int64_t) (bytes / 16);
+        int dev = 0, cus = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        const unsigned grid = (unsigned) (cus > 0 ? cus * 16 : 4096);
+        k_copy<<<grid, 256, 0, stream()>>>(static_cast<const double2*>(a), static_cast<double2*>(b), n);
+        hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
+        for (int i = 0; i < iters; i++)
+            k_copy<<<grid, 256, 0, stream()>>>(static_cast<const double2*>(a), static_cast<double2*>(b), n);
+        hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        gbps = 2.0 * (double) bytes * iters / (ms * 1e-3) / 1e9;
+    } catch (const Error& e) {
+        report("armon_copy_bandwidth", e);
+    }
+    if (e0) (void) hipEventDestroy(e0);
+    if (e1) (void) hipEventDestroy(e1);
+    if (a) (void) hipFree(a);
+    if (b) (void) hipFree(b);
+    return gbps;
+}
+
 // utils.cpp:20-27
 ARMON_EXPORT int idx_size(void) { return sizeof(Idx); }
 ARMON_EXPORT bool is_uidx_signed(void) { return false; }

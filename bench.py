@@ -98,6 +98,8 @@ def main():
     n = args.cells
     cells = n * n
 
+    copy_gbps = armon.load().armon_copy_bandwidth(1 << 31, 20)
+    log(f"[bench] device copy bandwidth (2 GiB buffers): {copy_gbps:.0f} GB/s")
     log(f"[bench] FP64 fused, Sedov {n}x{n}, warmup {args.warmup}, steps {args.steps}")
     el64, kt64, rho64 = measure_gpu(armon, n, n, args.steps, args.warmup, 0, 10)
     rate64 = cells * args.steps / el64
@@ -156,7 +158,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None, "kernel": names[dom],
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
-                     "kernel_avg_ms": sweep_avg},
+                     "kernel_avg_ms": sweep_avg, "copy_GBps": copy_gbps},
         "fp32_mixed": fp32,
         "cpu_baseline": cpu,
     }

@@ -57,6 +57,13 @@ ARMON_EXPORT void dt_CFL_device(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D
                                 ARMON_CREF(view) umat, ARMON_CREF(view) vmat, ARMON_CREF(view) cmat,
                                 ARMON_CREF(mask_view) domain_mask, double* dt_device);
 
+/* ---- measurement --------------------------------------------------------- */
+
+/* Bandwidth (GB/s, bytes read + written per second) of a plain device copy of `bytes`
+ * bytes repeated `iters` times: the achievable HBM rate the sweeps are compared with.
+ * Returns a negative value on error. */
+ARMON_EXPORT double armon_copy_bandwidth(size_t bytes, int iters);
+
 /* ---- parameters (src/parameters.h:17-108) -------------------------------- */
 
 enum { ARMON_SCHEME_GODUNOV = 0, ARMON_SCHEME_GAD = 1 };

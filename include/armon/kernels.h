@@ -91,47 +91,47 @@ enum { Limiter_None = 0, Limiter_Minmod = 1, Limiter_Superbee = 2 };
 enum { Test_Sod = 0, Test_Sod_y = 1, Test_Sod_circ = 2, Test_Bizarrium = 3, Test_Sedov = 4 };
 #endif
 
-/* acoustic.cpp:22-37 — first-order acoustic Godunov flux at the faces of `range` */
+/* Replaces kernels.h:10-13 (body acoustic.cpp:22-37): first-order acoustic Godunov flux at the faces of `range` */
 ARMON_EXPORT void acoustic(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, Idx s,
                            ARMON_REF(view) ustar, ARMON_REF(view) pstar,
                            ARMON_CREF(view) rho, ARMON_CREF(view) u, ARMON_CREF(view) pmat, ARMON_CREF(view) cmat);
 
-/* acoustic.cpp:91-108 — second-order GAD flux with a None/Minmod/Superbee limiter */
+/* Replaces kernels.h:15-19 (body acoustic.cpp:40-108): second-order GAD flux, None/Minmod/Superbee limiter */
 ARMON_EXPORT void acoustic_GAD(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, Idx s, flt_t dt, flt_t dx,
                                ARMON_REF(view) ustar, ARMON_REF(view) pstar,
                                ARMON_CREF(view) rho, ARMON_CREF(view) u, ARMON_CREF(view) pmat, ARMON_CREF(view) cmat,
                                Limiter limiter);
 
-/* fluid_eos.cpp:9-23 — perfect gas EOS */
+/* Replaces kernels.h:21-24 (body fluid_eos.cpp:9-23): perfect gas EOS */
 ARMON_EXPORT void perfect_gas_EOS(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, flt_t gamma,
                                   ARMON_CREF(view) rho, ARMON_CREF(view) Emat, ARMON_CREF(view) umat, ARMON_CREF(view) vmat,
                                   ARMON_REF(view) pmat, ARMON_REF(view) cmat, ARMON_REF(view) gmat);
 
-/* fluid_eos.cpp:26-62 — Bizarrium analytic EOS */
+/* Replaces kernels.h:26-29 (body fluid_eos.cpp:26-62): Bizarrium analytic EOS */
 ARMON_EXPORT void bizarrium_EOS(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range,
                                 ARMON_CREF(view) rho, ARMON_CREF(view) umat, ARMON_CREF(view) vmat, ARMON_CREF(view) Emat,
                                 ARMON_REF(view) pmat, ARMON_REF(view) cmat, ARMON_REF(view) gmat);
 
-/* euler_projection.cpp:9-22 — Lagrangian cell update, in place */
+/* Replaces kernels.h:31-33 (body euler_projection.cpp:9-22): Lagrangian cell update, in place */
 ARMON_EXPORT void cell_update(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, Idx s, flt_t dx, flt_t dt,
                               ARMON_CREF(view) ustar, ARMON_CREF(view) pstar,
                               ARMON_REF(view) rho, ARMON_REF(view) u, ARMON_REF(view) Emat);
 
-/* euler_projection.cpp:25-48 — remap of the conservative variables, in place */
+/* Replaces kernels.h:35-39 (body euler_projection.cpp:25-48): remap of the conservative variables, in place */
 ARMON_EXPORT void euler_projection(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, Idx s, flt_t dx, flt_t dt,
                                    ARMON_CREF(view) ustar, ARMON_REF(view) rho, ARMON_REF(view) umat,
                                    ARMON_REF(view) vmat, ARMON_REF(view) Emat,
                                    ARMON_CREF(view) advection_rho, ARMON_CREF(view) advection_urho,
                                    ARMON_CREF(view) advection_vrho, ARMON_CREF(view) advection_Erho);
 
-/* euler_projection.cpp:51-70 — first-order (donor cell) advection fluxes */
+/* Replaces kernels.h:41-44 (body euler_projection.cpp:51-70): first-order (donor cell) advection fluxes */
 ARMON_EXPORT void advection_first_order(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, Idx s, flt_t dt,
                                         ARMON_CREF(view) ustar, ARMON_CREF(view) rho, ARMON_CREF(view) umat,
                                         ARMON_CREF(view) vmat, ARMON_CREF(view) Emat,
                                         ARMON_REF(view) advection_rho, ARMON_REF(view) advection_urho,
                                         ARMON_REF(view) advection_vrho, ARMON_REF(view) advection_Erho);
 
-/* euler_projection.cpp:82-120 — second-order (minmod slope) advection fluxes */
+/* Replaces kernels.h:46-49 (body euler_projection.cpp:73-120): second-order (minmod slope) advection fluxes */
 ARMON_EXPORT void advection_second_order(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range, Idx s,
                                          flt_t dx, flt_t dt,
                                          ARMON_CREF(view) ustar, ARMON_CREF(view) rho, ARMON_CREF(view) umat,
@@ -139,28 +139,28 @@ ARMON_EXPORT void advection_second_order(ARMON_CREF(Range) range, ARMON_CREF(Inn
                                          ARMON_REF(view) advection_rho, ARMON_REF(view) advection_urho,
                                          ARMON_REF(view) advection_vrho, ARMON_REF(view) advection_Erho);
 
-/* halo_exchange.cpp:9-36 — mirror ghost fill of `stencil_width` layers */
+/* Replaces kernels.h:51-55 (body halo_exchange.cpp:9-36): mirror ghost fill of `stencil_width` layers */
 ARMON_EXPORT void boundary_conditions(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D) inner_range,
                                       Idx stencil_width, Idx disp, flt_t u_factor, flt_t v_factor,
                                       ARMON_REF(view) rho, ARMON_REF(view) umat, ARMON_REF(view) vmat,
                                       ARMON_REF(view) pmat, ARMON_REF(view) cmat, ARMON_REF(view) gmat,
                                       ARMON_REF(view) Emat);
 
-/* halo_exchange.cpp:39-64 — pack 7 fields x nghost layers of a border */
+/* Replaces kernels.h:57-62 (body halo_exchange.cpp:39-64): pack 7 fields x nghost layers of a border */
 ARMON_EXPORT void read_border_array(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range,
                                     Idx side_length, Idx nghost,
                                     ARMON_CREF(view) rho, ARMON_CREF(view) umat, ARMON_CREF(view) vmat,
                                     ARMON_CREF(view) pmat, ARMON_CREF(view) cmat, ARMON_CREF(view) gmat,
                                     ARMON_CREF(view) Emat, ARMON_REF(view) value_array);
 
-/* halo_exchange.cpp:67-92 — unpack 7 fields x nghost layers into a border */
+/* Replaces kernels.h:64-69 (body halo_exchange.cpp:67-92): unpack 7 fields x nghost layers into a border */
 ARMON_EXPORT void write_border_array(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range,
                                      Idx side_length, Idx nghost,
                                      ARMON_REF(view) rho, ARMON_REF(view) umat, ARMON_REF(view) vmat,
                                      ARMON_REF(view) pmat, ARMON_REF(view) cmat, ARMON_REF(view) gmat,
                                      ARMON_REF(view) Emat, ARMON_CREF(view) value_array);
 
-/* init_test.cpp:107-143 — test case initial state over `range` */
+/* Replaces kernels.h:71-77 (body init_test.cpp:51-143): test case initial state over `range` */
 ARMON_EXPORT void init_test(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range,
                             Idx row_length, Idx nb_ghosts, Idx nx, Idx ny, flt_t sx,
                             flt_t sy, flt_t ox, flt_t oy, Idx pos_x, Idx pos_y, Idx g_nx, Idx g_ny,
@@ -170,21 +170,21 @@ ARMON_EXPORT void init_test(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) in
                             ARMON_REF(view) ustar, ARMON_REF(view) pstar,
                             Test test, bool debug_indexes, flt_t test_option);
 
-/* reductions.cpp:8-24 — CFL time step (min reduction), returned to the host */
+/* Replaces kernels.h:79-81 (body reductions.cpp:8-24): CFL time step (min reduction), returned to the host */
 ARMON_EXPORT flt_t dt_CFL(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D) inner_range, flt_t dx, flt_t dy,
                           ARMON_CREF(view) umat, ARMON_CREF(view) vmat, ARMON_CREF(view) cmat,
                           ARMON_CREF(mask_view) domain_mask);
 
-/* reductions.cpp:27-49 — total mass and energy (compensated sums) */
+/* Replaces kernels.h:83-86 (body reductions.cpp:27-49): total mass and energy (compensated sums) */
 ARMON_EXPORT void conservation_vars(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D) inner_range, flt_t dx,
                                     ARMON_CREF(view) rho, ARMON_CREF(view) Emat, ARMON_CREF(mask_view) domain_mask,
                                     ARMON_REF(flt_t) total_mass, ARMON_REF(flt_t) total_energy);
 
-/* utils.cpp:16-17 — error funnel. The default prints and exits(1). A
+/* Replaces utils.h:99-100 / utils.cpp:16-17: error funnel. The default prints and exits(1). A
  * replacement may return: the entry point then returns without launching. */
 ARMON_EXPORT extern void (*raise_exception_handler)(const char* kernel, const char* msg);
 
-/* utils.cpp:20-27 — ABI queries */
+/* Replace utils.cpp:20-27: ABI queries */
 ARMON_EXPORT int idx_size(void);
 ARMON_EXPORT bool is_uidx_signed(void);
 ARMON_EXPORT int flt_size(void);
