@@ -42,6 +42,14 @@ ARMON_HD double cr_pow4(double y)
     return q + (e2 + t);
 }
 
+// Division. FP64 (and FP32 outside the FP32-mixed kernels) is the IEEE quotient a / b. The
+// FP32-mixed translation unit, which is judged by accuracy rather than bit parity, uses the
+// native reciprocal instead (about 1 ulp).
+template <class T> ARMON_HD T dv(T a, T b) { return a / b; }
+#if defined(ARMON_FAST_FP32) && defined(__HIP_DEVICE_COMPILE__)
+template <> ARMON_HD float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+#endif
+
 template <class T> ARMON_HD T sqrt_t(T x);
 template <> ARMON_HD double sqrt_t<double>(double x) { return __builtin_sqrt(x); }
 template <> ARMON_HD float sqrt_t<float>(float x) { return __builtin_sqrtf(x); }
@@ -76,8 +84,8 @@ ARMON_HD void godunov(T rho_i, T rho_im, T c_i, T c_im, T u_i, T u_im, T p_i, T 
 {
     T rc_l = rho_im * c_im;
     T rc_r = rho_i * c_i;
-    ustar = (rc_l * u_im + rc_r * u_i + (p_im - p_i)) / (rc_l + rc_r);
-    pstar = (rc_r * p_im + rc_l * p_i + rc_l * rc_r * (u_im - u_i)) / (rc_l + rc_r);
+    ustar = dv<T>(rc_l * u_im + rc_r * u_i + (p_im - p_i), rc_l + rc_r);
+    pstar = dv<T>(rc_r * p_im + rc_l * p_i + rc_l * rc_r * (u_im - u_i), rc_l + rc_r);
 }
 
 // acoustic.cpp:65-86: GAD second-order correction at face i (between cells i-1 and i),
@@ -89,10 +97,10 @@ ARMON_HD void gad_correct(T dt, T dx,
                           T ustar_im, T pstar_im, T ustar_i, T pstar_i, T ustar_ip, T pstar_ip,
                           T& ustar, T& pstar)
 {
-    T r_um = (ustar_ip - u_c) / (ustar_i - u_m + T(1e-6));
-    T r_pm = (pstar_ip - p_c) / (pstar_i - p_m + T(1e-6));
-    T r_up = (u_m - ustar_im) / (u_c - ustar_i + T(1e-6));
-    T r_pp = (p_m - pstar_im) / (p_c - pstar_i + T(1e-6));
+    T r_um = dv<T>(ustar_ip - u_c, ustar_i - u_m + T(1e-6));
+    T r_pm = dv<T>(pstar_ip - p_c, pstar_i - p_m + T(1e-6));
+    T r_up = dv<T>(u_m - ustar_im, u_c - ustar_i + T(1e-6));
+    T r_pp = dv<T>(p_m - pstar_im, p_c - pstar_i + T(1e-6));
 
     r_um = limiter<L>(r_um);
     r_pm = limiter<L>(r_pm);
@@ -105,7 +113,7 @@ ARMON_HD void gad_correct(T dt, T dx,
 
     T rc_l = rho_m * c_m;
     T rc_r = rho_c * c_c;
-    T theta = T(0.5) * (1 - (rc_l + rc_r) / 2 * (dt / Dm));
+    T theta = T(0.5) * (1 - (rc_l + rc_r) / 2 * dv<T>(dt, Dm));
 
     ustar = ustar_i + theta * (r_up * (u_c - ustar_i) - r_um * (ustar_i - u_m));
     pstar = pstar_i + theta * (r_pp * (p_c - pstar_i) - r_pm * (pstar_i - p_m));
@@ -117,7 +125,7 @@ ARMON_HD void perfect_gas(T gamma, T rho, T E, T u, T v, T& p, T& c, T& g)
 {
     T e = E - T(0.5) * (u * u + v * v);
     p = (gamma - 1) * rho * e;
-    c = sqrt_t<T>(gamma * p / rho);
+    c = sqrt_t<T>(dv<T>(gamma * p, rho));
     g = (1 + gamma) / 2;
 }
 
@@ -159,9 +167,9 @@ template <class T>
 ARMON_HD void cell_update(T dx, T dt, T us_i, T us_ip, T ps_i, T ps_ip, T& rho, T& u, T& E)
 {
     T dm = rho * dx;
-    rho = dm / (dx + dt * (us_ip - us_i));
-    u += dt / dm * (ps_i - ps_ip);
-    E += dt / dm * (ps_i * us_i - ps_ip * us_ip);
+    rho = dv<T>(dm, dx + dt * (us_ip - us_i));
+    u += dv<T>(dt, dm) * (ps_i - ps_ip);
+    E += dv<T>(dt, dm) * (ps_i * us_i - ps_ip * us_ip);
 }
 
 // euler_projection.cpp:73-79
@@ -199,15 +207,15 @@ ARMON_HD void advection_flux_2(T dx, T dt, T disp, T Dx,
     T Dx_l = dx + dt * (us_p - us_c);
     T Dx_lp = dx + dt * (us_pp - us_p);
 
-    T r_m = (2 * Dx_l) / (Dx_l + Dx_lm);
-    T r_p = (2 * Dx_l) / (Dx_l + Dx_lp);
+    T r_m = dv<T>(2 * Dx_l, Dx_l + Dx_lm);
+    T r_p = dv<T>(2 * Dx_l, Dx_l + Dx_lp);
 
     T slope_r = slope_minmod(rho_m, rho_c, rho_p, r_m, r_p);
     T slope_ur = slope_minmod(rho_m * u_m, rho_c * u_c, rho_p * u_p, r_m, r_p);
     T slope_vr = slope_minmod(rho_m * v_m, rho_c * v_c, rho_p * v_p, r_m, r_p);
     T slope_Er = slope_minmod(rho_m * E_m, rho_c * E_c, rho_p * E_p, r_m, r_p);
 
-    T length_factor = Dx / (2 * Dx_l);
+    T length_factor = dv<T>(Dx, 2 * Dx_l);
     f_rho = disp * (rho_c - slope_r * length_factor);
     f_urho = disp * (rho_c * u_c - slope_ur * length_factor);
     f_vrho = disp * (rho_c * v_c - slope_vr * length_factor);
@@ -232,15 +240,15 @@ ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
         n_Erho = (dX * rho * E - (f_Erho_ip - f_Erho_i)) * inv_dx;
     } else {
         (void) inv_dx;
-        n_rho = (dX * rho - (f_rho_ip - f_rho_i)) / dx;
-        n_urho = (dX * rho * u - (f_urho_ip - f_urho_i)) / dx;
-        n_vrho = (dX * rho * v - (f_vrho_ip - f_vrho_i)) / dx;
-        n_Erho = (dX * rho * E - (f_Erho_ip - f_Erho_i)) / dx;
+        n_rho = dv<T>(dX * rho - (f_rho_ip - f_rho_i), dx);
+        n_urho = dv<T>(dX * rho * u - (f_urho_ip - f_urho_i), dx);
+        n_vrho = dv<T>(dX * rho * v - (f_vrho_ip - f_vrho_i), dx);
+        n_Erho = dv<T>(dX * rho * E - (f_Erho_ip - f_Erho_i), dx);
     }
     rho = n_rho;
-    u = n_urho / n_rho;
-    v = n_vrho / n_rho;
-    E = n_Erho / n_rho;
+    u = dv<T>(n_urho, n_rho);
+    v = dv<T>(n_vrho, n_rho);
+    E = dv<T>(n_Erho, n_rho);
 }
 
 // reductions.cpp:18-20: CFL bound of one cell (mask 1 for real cells).
@@ -249,7 +257,7 @@ ARMON_HD T cfl_cell(T dx, T dy, T u, T v, T c, T mask)
 {
     T max_cx = kmax(abs_t<T>(u + c), abs_t<T>(u - c)) * mask;
     T max_cy = kmax(abs_t<T>(v + c), abs_t<T>(v - c)) * mask;
-    return kmin(dx / max_cx, dy / max_cy);
+    return kmin(dv<T>(dx, max_cx), dv<T>(dy, max_cy));
 }
 
 }  // namespace armon

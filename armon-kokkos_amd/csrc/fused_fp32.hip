@@ -1,0 +1,12 @@
+// Fused engine, FP32-mixed instantiation of the sweep kernels (sweep_kernels.hpp).
+//
+// Compiled with FMA contraction and with the native-rate FP32 reciprocal and square root:
+// this form is judged by its L1/Linf error against the FP64 engine, not by bit parity.
+// The EOS, the CFL bound and the conservation sums stay in FP64.
+#include "sweep_kernels.hpp"
+
+namespace armon {
+
+template void launch_sweep<float>(int, const SweepVariant&, const SweepArgs<float>&, hipStream_t);
+
+}  // namespace armon

@@ -1,0 +1,478 @@
+// Fused Lagrange + remap sweep kernels (templates), instantiated by fused.hip for FP64 and
+// by fused_fp32.hip for the FP32-mixed form, each with its own floating-point flags.
+//
+// One kernel per axis sweep performs, for every real cell, the whole per-axis chain of the
+// reference (armon_2D.cpp:309-320): EOS -> boundary ghost synthesis -> Godunov / GAD face
+// fluxes -> Lagrangian cell update -> first/second-order advection fluxes -> Euler
+// projection, reading the state once and writing it once (out of place, ping-pong buffers).
+//
+//  * X sweep (k_sweep_x): one 256-thread block per row tile. Each thread owns one cell of
+//    the tile (outputs plus an R-cell halo on each side, R = ghost layers read); the stage
+//    results are exchanged through LDS between barriers. Loads and stores are contiguous.
+//  * Y sweep (k_sweep_y): one thread per column, marching down a run of rows. Every
+//    stage's stencil neighbours along y live in the thread's own registers (a sliding
+//    window), so no LDS and no barriers; consecutive lanes touch consecutive columns.
+//
+// Ghost cells are synthesised in registers: layers within the stencil width mirror the real
+// cells with the side's velocity factors (halo_exchange.cpp:9-36); deeper layers keep their
+// never-refilled values from the arrays, as in the reference.
+//
+// The CFL reduction of the next cycle (reductions.cpp:8-24) is folded into the epilogue of
+// the last sweep of a cycle: it needs the new velocities and the sound speed of that sweep's
+// input state, which is exactly what the reference's dt_CFL reads at the start of the next
+// cycle (cmat is last written by the EOS of the last sweep).
+#pragma once
+
+#include "fused.hpp"
+#include "device_math.hpp"
+
+#include <cfloat>
+#include <stdexcept>
+#include <string>
+
+namespace armon {
+namespace sweep {
+
+constexpr int kTile = 256;    // X sweep: threads (cells) per tile
+constexpr int kYBlock = 256;  // Y sweep: columns per block
+
+template <class T>
+struct Cell {
+    T rho, un, ut, E, p, c;   // un: velocity along the sweep axis, ut: across it
+};
+
+__device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long) b); }
+__device__ __forceinline__ unsigned long long double_to_bits(double d) { return (unsigned long long) __double_as_longlong(d); }
+
+__device__ __forceinline__ bool finite_d(double x)
+{
+    return (double_to_bits(x) & 0x7FF0000000000000ull) != 0x7FF0000000000000ull;
+}
+
+// EOS of one cell (fluid_eos.cpp:9-62). The perfect gas is evaluated in the working
+// precision T; the Bizarrium EOS always in FP64, because its `e - eps_k0` and `p - pk0`
+// terms cancel to a few digits near the reference density (p and c rounded to T).
+template <class T, int EOS>
+__device__ __forceinline__ void cell_eos(double gamma, T rho, T u, T v, T E, T& p, T& c)
+{
+    if constexpr (EOS == 0) {
+        T g;
+        perfect_gas<T>((T) gamma, rho, E, u, v, p, c, g);
+    } else {
+        double pd, cd, gd;
+        bizarrium((double) rho, (double) E, (double) u, (double) v, pd, cd, gd);
+        p = (T) pd;
+        c = (T) cd;
+    }
+}
+
+// Cell `k` (along the sweep axis, `n` real cells) of pencil `q` (row for X, column for Y).
+// Real cells are loaded; ghost layers 1..stencil_width mirror real cells with the side's
+// factors; deeper ghost layers are read as they are, with p and c from stale_p/stale_c.
+template <class T, int AXIS, int EOS>
+__device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k, int n)
+{
+    int kk = k;
+    int mirror = 0;   // 0 real, 1 low side, 2 high side
+    bool stale = false;
+    if (k < 0) {
+        if (-k <= a.stencil_width) { kk = -k - 1; mirror = 1; }
+        else stale = true;
+    } else if (k >= n) {
+        if (k - n + 1 <= a.stencil_width) { kk = 2 * n - 1 - k; mirror = 2; }
+        else stale = true;
+    }
+    const int idx = AXIS == 0 ? a.index_start + q * a.row_length + kk : a.index_start + kk * a.row_length + q;
+    T rho = a.in_rho[idx], u = a.in_u[idx], v = a.in_v[idx], E = a.in_E[idx];
+    if (mirror == 1) {
+        u = u * (T) a.lo_u;
+        v = v * (T) a.lo_v;
+    } else if (mirror == 2) {
+        u = u * (T) a.hi_u;
+        v = v * (T) a.hi_v;
+    }
+    Cell<T> cell;
+    cell.rho = rho;
+    cell.E = E;
+    cell.un = AXIS == 0 ? u : v;
+    cell.ut = AXIS == 0 ? v : u;
+    if (stale) {
+        cell.p = a.stale_p[idx];
+        cell.c = a.stale_c[idx];
+    } else {
+        cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);
+    }
+    return cell;
+}
+
+template <class T>
+__device__ __forceinline__ Cell<T> benign_cell()
+{
+    Cell<T> c;
+    c.rho = T(1); c.un = T(0); c.ut = T(0); c.E = T(1); c.p = T(1); c.c = T(1);
+    return c;
+}
+
+template <class T, int L>
+__device__ __forceinline__ void face_flux_gad(T dt, T dx, const Cell<T>& m, const Cell<T>& c, T gu_im, T gp_im,
+                                              T gu_i, T gp_i, T gu_ip, T gp_ip, T& us, T& ps)
+{
+    gad_correct<L, T>(dt, dx, m.rho, c.rho, m.c, c.c, m.un, c.un, m.p, c.p, gu_im, gp_im, gu_i, gp_i, gu_ip, gp_ip,
+                      us, ps);
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = kmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Merges a per-lane CFL bound into the device accumulator: one atomic per wave.
+__device__ __forceinline__ void reduce_cfl(DeviceState* st, double local)
+{
+    local = wave_min(local);
+    if ((threadIdx.x & 63) == 0) atomicMin(&st->dt_min_bits, double_to_bits(local));
+}
+
+// ---------------------------------------------------------------------------------------
+// X sweep: LDS-staged row tiles
+// ---------------------------------------------------------------------------------------
+
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
+{
+    constexpr int GAD = FLUX > 0 ? 1 : 0;
+    constexpr int E2 = PROJ == 2 ? 1 : 0;
+    constexpr int R = 2 + GAD + E2;
+    constexpr int W = kTile - 2 * R;
+
+    if (!a.st->active) return;
+    const T dt = (T) (a.st->current_dt * a.dt_factor);
+    const T dx = (T) a.dx;
+
+    const int t = threadIdx.x;
+    const int j = blockIdx.x / nseg;
+    const int x0 = (blockIdx.x - j * nseg) * W;
+    const int k = x0 - R + t;
+    const bool in_range = k < a.nx + R;   // k >= -R always holds
+    const int tm = t > 0 ? t - 1 : 0;
+    const int tp = t < kTile - 1 ? t + 1 : kTile - 1;
+
+    __shared__ T s_rho[kTile], s_un[kTile], s_ut[kTile], s_E[kTile], s_p[kTile], s_c[kTile];
+    __shared__ T s_gu[kTile], s_gp[kTile], s_us[kTile], s_ps[kTile];
+    __shared__ T s_f[4][kTile];
+
+    // Stage 0: load (or synthesise) the cell, EOS.
+    const Cell<T> cell = in_range ? load_cell<T, 0, EOS>(a, j, k, a.nx) : benign_cell<T>();
+    s_rho[t] = cell.rho;
+    s_un[t] = cell.un;
+    s_p[t] = cell.p;
+    s_c[t] = cell.c;
+    __syncthreads();
+
+    // Stage 1-2: face flux at face k (between cells k-1 and k).
+    T gu, gp, us, ps;
+    godunov<T>(cell.rho, s_rho[tm], cell.c, s_c[tm], cell.un, s_un[tm], cell.p, s_p[tm], gu, gp);
+    if constexpr (GAD) {
+        s_gu[t] = gu;
+        s_gp[t] = gp;
+        __syncthreads();
+        Cell<T> m;
+        m.rho = s_rho[tm]; m.un = s_un[tm]; m.p = s_p[tm]; m.c = s_c[tm];
+        face_flux_gad<T, FLUX - 1>(dt, dx, m, cell, s_gu[tm], s_gp[tm], gu, gp, s_gu[tp], s_gp[tp], us, ps);
+    } else {
+        us = gu;
+        ps = gp;
+    }
+    s_us[t] = us;
+    s_ps[t] = ps;
+    __syncthreads();
+
+    // Stage 3: Lagrangian update of cell k from faces k and k+1.
+    T rho = cell.rho, un = cell.un, ut = cell.ut, E = cell.E;
+    const T us_p = s_us[tp];
+    cell_update<T>(dx, dt, us, us_p, ps, s_ps[tp], rho, un, E);
+    s_rho[t] = rho;
+    s_un[t] = un;
+    s_ut[t] = ut;
+    s_E[t] = E;
+    __syncthreads();
+
+    // Stage 4: advection flux at face k.
+    T f0, f1, f2, f3;
+    const T disp = dt * us;
+    if constexpr (E2) {
+        T Dx;
+        int td;
+        if (disp > 0) {
+            Dx = -(dx - dt * s_us[tm]);
+            td = tm;
+        } else {
+            Dx = dx + dt * s_us[tp];
+            td = t;
+        }
+        const int tdm = td > 0 ? td - 1 : 0;
+        const int tdp = td < kTile - 1 ? td + 1 : kTile - 1;
+        const int tdpp = td < kTile - 2 ? td + 2 : kTile - 1;
+        advection_flux_2<T>(dx, dt, disp, Dx, s_us[tdm], s_us[td], s_us[tdp], s_us[tdpp], s_rho[tdm], s_rho[td],
+                            s_rho[tdp], s_un[tdm], s_un[td], s_un[tdp], s_ut[tdm], s_ut[td], s_ut[tdp], s_E[tdm],
+                            s_E[td], s_E[tdp], f0, f1, f2, f3);
+    } else {
+        const int td = disp > 0 ? tm : t;
+        advection_flux_1<T>(disp, s_rho[td], s_un[td], s_ut[td], s_E[td], f0, f1, f2, f3);
+    }
+    s_f[0][t] = f0;
+    s_f[1][t] = f1;
+    s_f[2][t] = f2;
+    s_f[3][t] = f3;
+    __syncthreads();
+
+    // Stage 5: projection of cell k, output.
+    double local = DBL_MAX;
+    if (t >= R && t < R + W && k < a.nx) {
+        const T inv_dx = (T) a.inv_dx;
+        if (a.dx_pow2)
+            projection<true, T>(dx, inv_dx, dt, us, us_p, f0, s_f[0][tp], f1, s_f[1][tp], f2, s_f[2][tp], f3,
+                                s_f[3][tp], rho, un, ut, E);
+        else
+            projection<false, T>(dx, inv_dx, dt, us, us_p, f0, s_f[0][tp], f1, s_f[1][tp], f2, s_f[2][tp], f3,
+                                 s_f[3][tp], rho, un, ut, E);
+        const int idx = a.index_start + j * a.row_length + k;
+        a.out_rho[idx] = rho;
+        a.out_u[idx] = un;
+        a.out_v[idx] = ut;
+        a.out_E[idx] = E;
+        if (a.reduce) local = cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) un, (double) ut, (double) cell.c, 1.0);
+    }
+    if (a.reduce) reduce_cfl(a.st, local);
+}
+
+// ---------------------------------------------------------------------------------------
+// Y sweep: register march down columns
+// ---------------------------------------------------------------------------------------
+
+template <class T, int N>
+__device__ __forceinline__ void push(T (&w)[N], T v)
+{
+#pragma unroll
+    for (int i = 0; i < N - 1; i++) w[i] = w[i + 1];
+    w[N - 1] = v;
+}
+
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kYBlock) k_sweep_y(SweepArgs<T> a, int ncolblk)
+{
+    constexpr int GAD = FLUX > 0 ? 1 : 0;
+    constexpr int E2 = PROJ == 2 ? 1 : 0;
+    constexpr int R = 2 + GAD + E2;
+
+    if (!a.st->active) return;
+    const T dt = (T) (a.st->current_dt * a.dt_factor);
+    const T dx = (T) a.dx;
+    const T inv_dx = (T) a.inv_dx;
+
+    const int seg = blockIdx.x / ncolblk;
+    const int x = (blockIdx.x - seg * ncolblk) * kYBlock + threadIdx.x;
+    const bool lane_ok = x < a.nx;
+    const int y0 = seg * a.seg_len;
+    const int y1 = min(y0 + a.seg_len, a.ny);
+
+    // Windows, oldest first (see the index map in each stage below).
+    Cell<T> pre[3];                       // cells n-2, n-1, n (pre-update state + EOS)
+    T g_u[3], g_p[3];                     // Godunov at faces n-2, n-1, n
+    T f_u[5], f_p[2];                     // final face flux: u* at faces F-4..F, p* at F-1..F (F = n - GAD)
+    T q_rho[4], q_un[4], q_ut[4], q_E[4]; // post-update cells U-3..U (U = F - 1)
+    T h0[2], h1[2], h2[2], h3[2];         // advection fluxes at faces A-1, A (A = U - E2)
+    T cw[R + 1];                          // sound speed of cells n-R..n
+#pragma unroll
+    for (int i = 0; i < 3; i++) { pre[i] = benign_cell<T>(); g_u[i] = T(0); g_p[i] = T(0); }
+#pragma unroll
+    for (int i = 0; i < 5; i++) f_u[i] = T(0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) { q_rho[i] = T(1); q_un[i] = T(0); q_ut[i] = T(0); q_E[i] = T(1); }
+    f_p[0] = f_p[1] = T(0);
+    h0[0] = h0[1] = h1[0] = h1[1] = h2[0] = h2[1] = h3[0] = h3[1] = T(0);
+#pragma unroll
+    for (int i = 0; i <= R; i++) cw[i] = T(1);
+
+    double local = DBL_MAX;
+
+#pragma unroll 4
+    for (int n = y0 - R; n < y1 + R; ++n) {
+        const Cell<T> cn = lane_ok ? load_cell<T, 1, EOS>(a, x, n, a.ny) : benign_cell<T>();
+        pre[0] = pre[1];
+        pre[1] = pre[2];
+        pre[2] = cn;
+        push(cw, cn.c);
+
+        // Godunov at face n (cells n-1 | n).
+        T gu, gp;
+        godunov<T>(pre[2].rho, pre[1].rho, pre[2].c, pre[1].c, pre[2].un, pre[1].un, pre[2].p, pre[1].p, gu, gp);
+        push(g_u, gu);
+        push(g_p, gp);
+
+        // Final flux at face F: GAD at F = n-1 (cells n-2 | n-1, Godunov n-2, n-1, n), else F = n.
+        T us, ps;
+        if constexpr (GAD) {
+            face_flux_gad<T, FLUX - 1>(dt, dx, pre[0], pre[1], g_u[0], g_p[0], g_u[1], g_p[1], g_u[2], g_p[2], us,
+                                       ps);
+        } else {
+            us = gu;
+            ps = gp;
+        }
+        push(f_u, us);
+        push(f_p, ps);
+
+        // Lagrangian update of cell U = F-1 from faces U (f_*[3], f_p[0]) and F (f_*[4], f_p[1]).
+        const Cell<T>& pu = GAD ? pre[0] : pre[1];
+        T rho = pu.rho, un = pu.un, ut = pu.ut, E = pu.E;
+        cell_update<T>(dx, dt, f_u[3], f_u[4], f_p[0], f_p[1], rho, un, E);
+        push(q_rho, rho);
+        push(q_un, un);
+        push(q_ut, ut);
+        push(q_E, E);
+
+        // Advection flux at face A.
+        T fa0, fa1, fa2, fa3;
+        if constexpr (E2) {
+            // A = U-1 = F-2: u*[A] = f_u[2]; cells q[k] = U-3+k.
+            const T disp = dt * f_u[2];
+            if (disp > 0) {
+                const T Dx = -(dx - dt * f_u[1]);
+                // donor d = A-1 = q[1]; u* at d-1..d+2 = f_u[0..3]
+                advection_flux_2<T>(dx, dt, disp, Dx, f_u[0], f_u[1], f_u[2], f_u[3], q_rho[0], q_rho[1], q_rho[2],
+                                    q_un[0], q_un[1], q_un[2], q_ut[0], q_ut[1], q_ut[2], q_E[0], q_E[1], q_E[2], fa0,
+                                    fa1, fa2, fa3);
+            } else {
+                const T Dx = dx + dt * f_u[3];
+                // donor d = A = q[2]; u* at d-1..d+2 = f_u[1..4]
+                advection_flux_2<T>(dx, dt, disp, Dx, f_u[1], f_u[2], f_u[3], f_u[4], q_rho[1], q_rho[2], q_rho[3],
+                                    q_un[1], q_un[2], q_un[3], q_ut[1], q_ut[2], q_ut[3], q_E[1], q_E[2], q_E[3], fa0,
+                                    fa1, fa2, fa3);
+            }
+        } else {
+            // A = U = F-1: u*[A] = f_u[3]; donor A-1 = q[2] or A = q[3].
+            const T disp = dt * f_u[3];
+            if (disp > 0)
+                advection_flux_1<T>(disp, q_rho[2], q_un[2], q_ut[2], q_E[2], fa0, fa1, fa2, fa3);
+            else
+                advection_flux_1<T>(disp, q_rho[3], q_un[3], q_ut[3], q_E[3], fa0, fa1, fa2, fa3);
+        }
+        push(h0, fa0);
+        push(h1, fa1);
+        push(h2, fa2);
+        push(h3, fa3);
+
+        // Projection of cell P = A-1 = n-R from faces P (h*[0]) and A (h*[1]).
+        const int P = n - R;
+        if (P >= y0 && lane_ok) {
+            constexpr int QP = E2 ? 1 : 2;    // q index of cell P
+            constexpr int FP = E2 ? 1 : 2;    // f_u index of face P
+            T r = q_rho[QP], u = q_un[QP], v = q_ut[QP], e = q_E[QP];
+            if (a.dx_pow2)
+                projection<true, T>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1], h2[0], h2[1],
+                                    h3[0], h3[1], r, u, v, e);
+            else
+                projection<false, T>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1], h2[0],
+                                     h2[1], h3[0], h3[1], r, u, v, e);
+            const int idx = a.index_start + P * a.row_length + x;
+            a.out_rho[idx] = r;
+            a.out_u[idx] = v;   // ut is the x velocity in a Y sweep
+            a.out_v[idx] = u;
+            a.out_E[idx] = e;
+            if (a.reduce)
+                local = kmin(local, cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) v, (double) u, (double) cw[0], 1.0));
+        }
+    }
+    if (a.reduce) reduce_cfl(a.st, local);
+}
+
+inline unsigned grid_for(int64_t n, int block, int64_t max_blocks)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g > max_blocks) g = max_blocks;
+    if (g < 1) g = 1;
+    return (unsigned) g;
+}
+
+// Blocks of `block` threads of `kernel` resident on the whole device at once.
+inline int resident_blocks(const void* kernel, int block)
+{
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+// Rows per marching run: the column blocks times the runs per column fill one resident round,
+// but a run is kept at least 16 halo widths long so that its warm-up stays small.
+inline int y_run_rows(int ny, int ncolblk, int resident, int halo)
+{
+    int nseg = resident > 0 ? resident / (ncolblk > 0 ? ncolblk : 1) : 1;
+    if (nseg < 1) nseg = 1;
+    int len = (ny + nseg - 1) / nseg;
+    const int min_len = 16 * halo;
+    if (len < min_len) len = min_len;
+    if (len > ny) len = ny;
+    return len > 0 ? len : 1;
+}
+
+inline void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T, int F, int P, int E>
+void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
+{
+    constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
+    if (axis == 0) {
+        constexpr int W = kTile - 2 * R;
+        const int nseg = (a.nx + W - 1) / W;
+        k_sweep_x<T, F, P, E><<<dim3((unsigned) (nseg * a.ny)), dim3(kTile), 0, s>>>(a, nseg);
+        check_launch("sweep_x");
+    } else {
+        // One resident round of blocks: each block marches down a contiguous run of rows of
+        // its 256 columns, so the warm-up is paid once per run and no partial second round
+        // of blocks leaves CUs idle.
+        static int resident = 0;
+        if (resident == 0) resident = resident_blocks((const void*) k_sweep_y<T, F, P, E>, kYBlock);
+        const int ncolblk = (a.nx + kYBlock - 1) / kYBlock;
+        SweepArgs<T> b = a;
+        b.seg_len = y_run_rows(a.ny, ncolblk, resident, R);
+        const int nseg = (a.ny + b.seg_len - 1) / b.seg_len;
+        k_sweep_y<T, F, P, E><<<dim3((unsigned) (ncolblk * nseg)), dim3(kYBlock), 0, s>>>(b, ncolblk);
+        check_launch("sweep_y");
+    }
+}
+
+template <class T, int F, int P>
+void launch_eos_dispatch(int axis, int eos, const SweepArgs<T>& a, hipStream_t s)
+{
+    if (eos == 0) launch_variant<T, F, P, 0>(axis, a, s);
+    else launch_variant<T, F, P, 1>(axis, a, s);
+}
+
+template <class T, int F>
+void launch_proj_dispatch(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStream_t s)
+{
+    if (v.proj == 2) launch_eos_dispatch<T, F, 2>(axis, v.eos, a, s);
+    else launch_eos_dispatch<T, F, 1>(axis, v.eos, a, s);
+}
+
+}  // namespace sweep
+
+template <class T>
+void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStream_t s)
+{
+    switch (v.flux) {
+    case 0: sweep::launch_proj_dispatch<T, 0>(axis, v, a, s); break;
+    case 1: sweep::launch_proj_dispatch<T, 1>(axis, v, a, s); break;
+    case 2: sweep::launch_proj_dispatch<T, 2>(axis, v, a, s); break;
+    default: sweep::launch_proj_dispatch<T, 3>(axis, v, a, s); break;
+    }
+}
+
+}  // namespace armon
