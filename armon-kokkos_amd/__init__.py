@@ -92,6 +92,11 @@ class ArmonResult(C.Structure):
                 ("final_mass", C.c_double), ("final_energy", C.c_double)]
 
 
+class ArmonTestState(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("high_rho", "low_rho", "high_E", "low_E",
+                                          "high_u", "low_u", "high_v", "low_v")]
+
+
 class ArmonKernelTimes(C.Structure):
     _fields_ = [("launches", C.c_int * 4), ("total_ms", C.c_double * 4)]
 
@@ -136,6 +141,10 @@ EXTENSION_SYMBOLS = {
     "armon_clear_error": (None, []),
     "armon_version": (C.c_char_p, []),
     "armon_copy_bandwidth": (C.c_double, [C.c_size_t, C.c_int]),
+    "armon_reference_test_state": (None, [C.c_int, _d, C.POINTER(ArmonTestState)]),
+    "init_test_states": (None, [_R, _I2, _i64, _i64, _i64, _i64, _d, _d, _d, _d, _i64, _i64, _i64, _i64,
+                                _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, C.c_int, _d,
+                                C.POINTER(ArmonTestState)]),
     "dt_CFL_device": (None, [_R, _I1, _d, _d, _V, _V, _V, _V, C.c_void_p]),
     "armon_params_default": (None, [_PP]),
     "armon_params_init": (C.c_int, [_PP]),

@@ -17,9 +17,8 @@ namespace {
 // Time step (armon_2D.cpp:212-252 and the loop condition of :347)
 // ---------------------------------------------------------------------------------------
 
-__global__ void k_time_step(DeviceState* st, TimeStepConfig c)
+__device__ __forceinline__ void time_step(DeviceState* st, const TimeStepConfig& c)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (st->error || !(st->time < c.max_time && st->cycle < c.max_cycles)) {
         st->active = 0;
         return;
@@ -52,6 +51,17 @@ __global__ void k_time_step(DeviceState* st, TimeStepConfig c)
     st->time += current;
     st->cycle += 1;
     st->active = 1;
+}
+
+__global__ void k_time_step(DeviceState* st, TimeStepConfig c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) time_step(st, c);
+}
+
+__global__ void k_batch_time_step(DeviceState* states, const TimeStepConfig* cfgs, int n)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < n) time_step(states + d, cfgs[d]);
 }
 
 __global__ void k_reset_min(DeviceState* st) { st->dt_min_bits = double_to_bits(DBL_MAX); }
@@ -103,6 +113,9 @@ __global__ void __launch_bounds__(256) k_eos_real(const T* rho, const T* u, cons
 }  // namespace
 
 template void launch_sweep<double>(int, const SweepVariant&, const SweepArgs<double>&, hipStream_t);
+template void launch_batch_sweep<double>(int, const SweepVariant&, const BatchArgs<double>&, const void*, int, int,
+                                         hipStream_t);
+template int batch_resident_blocks<double>(int, const SweepVariant&);
 
 int x_tile_outputs(const SweepVariant& v) { return kTile - 2 * read_extent(v.flux, v.proj); }
 
@@ -110,6 +123,12 @@ void launch_time_step(DeviceState* st, const TimeStepConfig& c, hipStream_t s)
 {
     k_time_step<<<1, 64, 0, s>>>(st, c);
     check_launch("time_step");
+}
+
+void launch_batch_time_step(DeviceState* states, const TimeStepConfig* cfgs, int n, hipStream_t s)
+{
+    k_batch_time_step<<<(unsigned) ((n + 63) / 64), 64, 0, s>>>(states, cfgs, n);
+    check_launch("batch_time_step");
 }
 
 template <class T>

@@ -59,6 +59,47 @@ int x_tile_outputs(const SweepVariant& v);
 
 void launch_time_step(DeviceState* st, const TimeStepConfig& c, hipStream_t s);
 
+// ---- batched ensemble --------------------------------------------------------------------
+
+// One domain of a batch: its slice of the packed arrays and its per-axis constants.
+struct BatchDomain {
+    long long offset;              // first element of the domain in the packed arrays
+    int nx, ny, row_length, index_start;
+    int dx_pow2[2];                // per axis
+    double dx[2], inv_dx[2];       // per axis
+    double cfl_dx, cfl_dy;
+    double bc[4][2];               // per side (Left, Right, Top, Bottom): u, v factors
+};
+
+// Work items: an X tile (domain, row, first output column) or a Y run of one wave
+// (domain, first column, output rows [y0, y1)).
+struct BatchItemX { int domain, row, x0, pad; };
+struct BatchItemY { int domain, x0, y0, y1; };
+
+template <class T>
+struct BatchArgs {
+    const T* in_rho; const T* in_u; const T* in_v; const T* in_E;
+    T* out_rho; T* out_u; T* out_v; T* out_E;
+    const T* stale_p; const T* stale_c;
+    const BatchDomain* domains;
+    DeviceState* states;
+    int stencil_width, reduce;
+    double gamma, dt_factor;
+};
+
+// One launch sweeps every domain of the batch along `axis`: `grid` persistent blocks walk
+// the work items (BatchItemX for X, BatchItemY for Y).
+template <class T>
+void launch_batch_sweep(int axis, const SweepVariant& v, const BatchArgs<T>& b, const void* items, int nitems,
+                        int grid, hipStream_t s);
+
+// Blocks of the batched sweep kernel of this variant that fit on the device at once.
+template <class T>
+int batch_resident_blocks(int axis, const SweepVariant& v);
+
+// The time step of every domain (one thread each), per-domain configuration.
+void launch_batch_time_step(DeviceState* states, const TimeStepConfig* cfgs, int n, hipStream_t s);
+
 // CFL minimum of the real cells into st->dt_min_bits (after resetting it).
 template <class T>
 void launch_cfl_real(DeviceState* st, const T* u, const T* v, const T* c, int nx, int ny, int row_length,

@@ -907,14 +907,12 @@ ARMON_KERNEL_TRY
 }
 ARMON_KERNEL_CATCH("write_border_array")
 
-ARMON_EXPORT void init_test(const Range& range, const InnerRange2D& inner_range, Idx row_length, Idx nb_ghosts,
-                            Idx nx, Idx ny, flt_t sx, flt_t sy, flt_t ox, flt_t oy, Idx pos_x, Idx pos_y, Idx g_nx,
-                            Idx g_ny, view& x, view& y, view& rho, view& Emat, view& umat, view& vmat,
-                            mask_view& domain_mask, view& pmat, view& cmat, view& gmat, view& ustar, view& pstar,
-                            Test test, bool debug_indexes, flt_t test_option)
-ARMON_KERNEL_TRY
+static void launch_init_test(const char* name, const Range& range, const InnerRange2D& inner_range, Idx row_length,
+                             Idx nb_ghosts, Idx nx, Idx ny, flt_t sx, flt_t sy, flt_t ox, flt_t oy, Idx pos_x,
+                             Idx pos_y, Idx g_nx, Idx g_ny, view& x, view& y, view& rho, view& Emat, view& umat,
+                             view& vmat, mask_view& domain_mask, view& pmat, view& cmat, view& gmat, view& ustar,
+                             view& pstar, int t, bool debug_indexes, flt_t test_option, const TestState& st)
 {
-    const int t = static_cast<int>(test);
     if (t < 0 || t > 4) {
         char buf[160];
         snprintf(buf, sizeof buf, "Invalid test case index: %d, expected a value between 0 and 4", t);
@@ -953,13 +951,55 @@ ARMON_KERNEL_TRY
     a.r = test_option;
     a.test = t;
     a.debug_indexes = debug_indexes;
-    a.st = test_state(t, test_option);
+    a.st = st;
     k_init_test<<<grid_2d(it), kBlock, 0, stream()>>>(it, a, x.data, y.data, rho.data, Emat.data, umat.data,
                                                        vmat.data, domain_mask.data, pmat.data, cmat.data, gmat.data,
                                                        ustar.data, pstar.data);
-    after_launch("init_test");
+    after_launch(name);
+}
+
+ARMON_EXPORT void init_test(const Range& range, const InnerRange2D& inner_range, Idx row_length, Idx nb_ghosts,
+                            Idx nx, Idx ny, flt_t sx, flt_t sy, flt_t ox, flt_t oy, Idx pos_x, Idx pos_y, Idx g_nx,
+                            Idx g_ny, view& x, view& y, view& rho, view& Emat, view& umat, view& vmat,
+                            mask_view& domain_mask, view& pmat, view& cmat, view& gmat, view& ustar, view& pstar,
+                            Test test, bool debug_indexes, flt_t test_option)
+ARMON_KERNEL_TRY
+{
+    const int t = static_cast<int>(test);
+    launch_init_test("init_test", range, inner_range, row_length, nb_ghosts, nx, ny, sx, sy, ox, oy, pos_x, pos_y,
+                     g_nx, g_ny, x, y, rho, Emat, umat, vmat, domain_mask, pmat, cmat, gmat, ustar, pstar, t,
+                     debug_indexes, test_option, test_state(t, test_option));
 }
 ARMON_KERNEL_CATCH("init_test")
+
+ARMON_EXPORT void armon_reference_test_state(int test, double test_option, ArmonTestState* out)
+{
+    const TestState t = test_state(test, (flt_t) test_option);
+    *out = ArmonTestState{t.high_rho, t.low_rho, t.high_E, t.low_E, t.high_u, t.low_u, t.high_v, t.low_v};
+}
+
+ARMON_EXPORT void init_test_states(const Range& range, const InnerRange2D& inner_range, Idx row_length,
+                                   Idx nb_ghosts, Idx nx, Idx ny, flt_t sx, flt_t sy, flt_t ox, flt_t oy, Idx pos_x,
+                                   Idx pos_y, Idx g_nx, Idx g_ny, view& x, view& y, view& rho, view& Emat,
+                                   view& umat, view& vmat, mask_view& domain_mask, view& pmat, view& cmat,
+                                   view& gmat, view& ustar, view& pstar, Test test, flt_t test_option,
+                                   const ArmonTestState& states)
+ARMON_KERNEL_TRY
+{
+    TestState st{};
+    st.high_rho = (flt_t) states.high_rho;
+    st.low_rho = (flt_t) states.low_rho;
+    st.high_E = (flt_t) states.high_E;
+    st.low_E = (flt_t) states.low_E;
+    st.high_u = (flt_t) states.high_u;
+    st.low_u = (flt_t) states.low_u;
+    st.high_v = (flt_t) states.high_v;
+    st.low_v = (flt_t) states.low_v;
+    launch_init_test("init_test_states", range, inner_range, row_length, nb_ghosts, nx, ny, sx, sy, ox, oy, pos_x,
+                     pos_y, g_nx, g_ny, x, y, rho, Emat, umat, vmat, domain_mask, pmat, cmat, gmat, ustar, pstar,
+                     static_cast<int>(test), false, test_option, st);
+}
+ARMON_KERNEL_CATCH("init_test_states")
 
 static void launch_dt_CFL(const Range& range, const InnerRange1D& inner_range, flt_t dx, flt_t dy,
                           const view& umat, const view& vmat, const view& cmat, const mask_view& domain_mask,
@@ -1047,12 +1087,23 @@ ARMON_KERNEL_CATCH("conservation_vars")
 }  // extern "C"
 
 namespace {
-// Plain streaming copy (16-byte lanes, grid-stride): the achievable HBM bandwidth of this
-// device, against which the sweeps' traffic and rate are compared.
-__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ in, double2* __restrict__ out, int64_t n)
+// Streaming copy: the achievable HBM bandwidth of this device, against which the sweeps'
+// traffic and rate are compared. Each thread keeps four 16-byte loads in flight before its
+// stores (consecutive lanes on consecutive 16-byte words), with non-temporal stores.
+typedef double copy_word __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256) k_copy(const copy_word* __restrict__ in, copy_word* __restrict__ out, int64_t n)
 {
-    for (int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t) gridDim.x * blockDim.x)
-        out[k] = in[k];
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+    int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k + 3 * stride < n; k += 4 * stride) {
+        const copy_word a = in[k], b = in[k + stride], c = in[k + 2 * stride], d = in[k + 3 * stride];
+        __builtin_nontemporal_store(a, &out[k]);
+        __builtin_nontemporal_store(b, &out[k + stride]);
+        __builtin_nontemporal_store(c, &out[k + 2 * stride]);
+        __builtin_nontemporal_store(d, &out[k + 3 * stride]);
+    }
+    for (; k < n; k += stride) __builtin_nontemporal_store(in[k], &out[k]);
 }
 }  // namespace
 
@@ -1075,11 +1126,11 @@ ARMON_EXPORT double armon_copy_bandwidth(size_t bytes, int iters)
         int dev = 0, cus = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-        const unsigned grid = (unsigned) (cus > 0 ? cus * 16 : 4096);
-        k_copy<<<grid, 256, 0, stream()>>>(static_cast<const double2*>(a), static_cast<double2*>(b), n);
+        const unsigned grid = (unsigned) (cus > 0 ? cus * 8 : 2048);
+        k_copy<<<grid, 256, 0, stream()>>>(static_cast<const copy_word*>(a), static_cast<copy_word*>(b), n);
         hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
         for (int i = 0; i < iters; i++)
-            k_copy<<<grid, 256, 0, stream()>>>(static_cast<const double2*>(a), static_cast<double2*>(b), n);
+            k_copy<<<grid, 256, 0, stream()>>>(static_cast<const copy_word*>(a), static_cast<copy_word*>(b), n);
         hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
         hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
         float ms = 0;

@@ -29,6 +29,7 @@
 #include <cfloat>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace armon {
 namespace sweep {
@@ -138,64 +139,65 @@ __device__ __forceinline__ void reduce_cfl(DeviceState* st, double local)
 // X sweep: LDS-staged row tiles
 // ---------------------------------------------------------------------------------------
 
+// LDS of one X tile: the cell state, the Godunov and final face fluxes, the advection fluxes.
+template <class T>
+struct SmemX {
+    T rho[kTile], un[kTile], ut[kTile], E[kTile], p[kTile], c[kTile];
+    T gu[kTile], gp[kTile], us[kTile], ps[kTile];
+    T f[4][kTile];
+};
+
+// One X tile: row j, output cells [x0, x0 + W) of the domain described by `a`, with the
+// cycle's dt. Called by all threads of a kTile-thread block.
 template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
+__device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j, int x0, SmemX<T>& sm)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
     constexpr int E2 = PROJ == 2 ? 1 : 0;
     constexpr int R = 2 + GAD + E2;
     constexpr int W = kTile - 2 * R;
-
-    if (!a.st->active) return;
-    const T dt = (T) (a.st->current_dt * a.dt_factor);
     const T dx = (T) a.dx;
 
     const int t = threadIdx.x;
-    const int j = blockIdx.x / nseg;
-    const int x0 = (blockIdx.x - j * nseg) * W;
     const int k = x0 - R + t;
     const bool in_range = k < a.nx + R;   // k >= -R always holds
     const int tm = t > 0 ? t - 1 : 0;
     const int tp = t < kTile - 1 ? t + 1 : kTile - 1;
 
-    __shared__ T s_rho[kTile], s_un[kTile], s_ut[kTile], s_E[kTile], s_p[kTile], s_c[kTile];
-    __shared__ T s_gu[kTile], s_gp[kTile], s_us[kTile], s_ps[kTile];
-    __shared__ T s_f[4][kTile];
-
     // Stage 0: load (or synthesise) the cell, EOS.
     const Cell<T> cell = in_range ? load_cell<T, 0, EOS>(a, j, k, a.nx) : benign_cell<T>();
-    s_rho[t] = cell.rho;
-    s_un[t] = cell.un;
-    s_p[t] = cell.p;
-    s_c[t] = cell.c;
+    sm.rho[t] = cell.rho;
+    sm.un[t] = cell.un;
+    sm.p[t] = cell.p;
+    sm.c[t] = cell.c;
     __syncthreads();
 
     // Stage 1-2: face flux at face k (between cells k-1 and k).
     T gu, gp, us, ps;
-    godunov<T>(cell.rho, s_rho[tm], cell.c, s_c[tm], cell.un, s_un[tm], cell.p, s_p[tm], gu, gp);
+    godunov<T>(cell.rho, sm.rho[tm], cell.c, sm.c[tm], cell.un, sm.un[tm], cell.p, sm.p[tm], gu, gp);
     if constexpr (GAD) {
-        s_gu[t] = gu;
-        s_gp[t] = gp;
+        sm.gu[t] = gu;
+        sm.gp[t] = gp;
         __syncthreads();
         Cell<T> m;
-        m.rho = s_rho[tm]; m.un = s_un[tm]; m.p = s_p[tm]; m.c = s_c[tm];
-        face_flux_gad<T, FLUX - 1>(dt, dx, m, cell, s_gu[tm], s_gp[tm], gu, gp, s_gu[tp], s_gp[tp], us, ps);
+        m.rho = sm.rho[tm]; m.un = sm.un[tm]; m.p = sm.p[tm]; m.c = sm.c[tm];
+        face_flux_gad<T, FLUX - 1>(dt, dx, m, cell, sm.gu[tm], sm.gp[tm], gu, gp, sm.gu[tp], sm.gp[tp], us, ps);
     } else {
         us = gu;
         ps = gp;
     }
-    s_us[t] = us;
-    s_ps[t] = ps;
+    sm.us[t] = us;
+    sm.ps[t] = ps;
     __syncthreads();
 
     // Stage 3: Lagrangian update of cell k from faces k and k+1.
     T rho = cell.rho, un = cell.un, ut = cell.ut, E = cell.E;
-    const T us_p = s_us[tp];
-    cell_update<T>(dx, dt, us, us_p, ps, s_ps[tp], rho, un, E);
-    s_rho[t] = rho;
-    s_un[t] = un;
-    s_ut[t] = ut;
-    s_E[t] = E;
+    const T us_p = sm.us[tp];
+    cell_update<T>(dx, dt, us, us_p, ps, sm.ps[tp], rho, un, E);
+    sm.rho[t] = rho;
+    sm.un[t] = un;
+    sm.ut[t] = ut;
+    sm.E[t] = E;
     __syncthreads();
 
     // Stage 4: advection flux at face k.
@@ -205,26 +207,26 @@ __global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
         T Dx;
         int td;
         if (disp > 0) {
-            Dx = -(dx - dt * s_us[tm]);
+            Dx = -(dx - dt * sm.us[tm]);
             td = tm;
         } else {
-            Dx = dx + dt * s_us[tp];
+            Dx = dx + dt * sm.us[tp];
             td = t;
         }
         const int tdm = td > 0 ? td - 1 : 0;
         const int tdp = td < kTile - 1 ? td + 1 : kTile - 1;
         const int tdpp = td < kTile - 2 ? td + 2 : kTile - 1;
-        advection_flux_2<T>(dx, dt, disp, Dx, s_us[tdm], s_us[td], s_us[tdp], s_us[tdpp], s_rho[tdm], s_rho[td],
-                            s_rho[tdp], s_un[tdm], s_un[td], s_un[tdp], s_ut[tdm], s_ut[td], s_ut[tdp], s_E[tdm],
-                            s_E[td], s_E[tdp], f0, f1, f2, f3);
+        advection_flux_2<T>(dx, dt, disp, Dx, sm.us[tdm], sm.us[td], sm.us[tdp], sm.us[tdpp], sm.rho[tdm], sm.rho[td],
+                            sm.rho[tdp], sm.un[tdm], sm.un[td], sm.un[tdp], sm.ut[tdm], sm.ut[td], sm.ut[tdp], sm.E[tdm],
+                            sm.E[td], sm.E[tdp], f0, f1, f2, f3);
     } else {
         const int td = disp > 0 ? tm : t;
-        advection_flux_1<T>(disp, s_rho[td], s_un[td], s_ut[td], s_E[td], f0, f1, f2, f3);
+        advection_flux_1<T>(disp, sm.rho[td], sm.un[td], sm.ut[td], sm.E[td], f0, f1, f2, f3);
     }
-    s_f[0][t] = f0;
-    s_f[1][t] = f1;
-    s_f[2][t] = f2;
-    s_f[3][t] = f3;
+    sm.f[0][t] = f0;
+    sm.f[1][t] = f1;
+    sm.f[2][t] = f2;
+    sm.f[3][t] = f3;
     __syncthreads();
 
     // Stage 5: projection of cell k, output.
@@ -232,11 +234,11 @@ __global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
     if (t >= R && t < R + W && k < a.nx) {
         const T inv_dx = (T) a.inv_dx;
         if (a.dx_pow2)
-            projection<true, T>(dx, inv_dx, dt, us, us_p, f0, s_f[0][tp], f1, s_f[1][tp], f2, s_f[2][tp], f3,
-                                s_f[3][tp], rho, un, ut, E);
+            projection<true, T>(dx, inv_dx, dt, us, us_p, f0, sm.f[0][tp], f1, sm.f[1][tp], f2, sm.f[2][tp], f3,
+                                sm.f[3][tp], rho, un, ut, E);
         else
-            projection<false, T>(dx, inv_dx, dt, us, us_p, f0, s_f[0][tp], f1, s_f[1][tp], f2, s_f[2][tp], f3,
-                                 s_f[3][tp], rho, un, ut, E);
+            projection<false, T>(dx, inv_dx, dt, us, us_p, f0, sm.f[0][tp], f1, sm.f[1][tp], f2, sm.f[2][tp], f3,
+                                 sm.f[3][tp], rho, un, ut, E);
         const int idx = a.index_start + j * a.row_length + k;
         a.out_rho[idx] = rho;
         a.out_u[idx] = un;
@@ -245,6 +247,17 @@ __global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
         if (a.reduce) local = cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) un, (double) ut, (double) cell.c, 1.0);
     }
     if (a.reduce) reduce_cfl(a.st, local);
+}
+
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
+{
+    constexpr int W = kTile - 2 * (2 + (FLUX > 0 ? 1 : 0) + (PROJ == 2 ? 1 : 0));
+    if (!a.st->active) return;
+    const T dt = (T) (a.st->current_dt * a.dt_factor);
+    __shared__ SmemX<T> sm;
+    const int j = blockIdx.x / nseg;
+    sweep_x_tile<T, FLUX, PROJ, EOS>(a, dt, j, (blockIdx.x - j * nseg) * W, sm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -259,23 +272,16 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
     w[N - 1] = v;
 }
 
+// One Y run: column x (lane_ok false for lanes past the domain), output rows [y0, y1),
+// with the cycle's dt. The whole wave must call it with the same y0, y1.
 template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kYBlock) k_sweep_y(SweepArgs<T> a, int ncolblk)
+__device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, bool lane_ok, int y0, int y1)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
     constexpr int E2 = PROJ == 2 ? 1 : 0;
     constexpr int R = 2 + GAD + E2;
-
-    if (!a.st->active) return;
-    const T dt = (T) (a.st->current_dt * a.dt_factor);
     const T dx = (T) a.dx;
     const T inv_dx = (T) a.inv_dx;
-
-    const int seg = blockIdx.x / ncolblk;
-    const int x = (blockIdx.x - seg * ncolblk) * kYBlock + threadIdx.x;
-    const bool lane_ok = x < a.nx;
-    const int y0 = seg * a.seg_len;
-    const int y1 = min(y0 + a.seg_len, a.ny);
 
     // Windows, oldest first (see the index map in each stage below).
     Cell<T> pre[3];                       // cells n-2, n-1, n (pre-update state + EOS)
@@ -387,6 +393,88 @@ __global__ void __launch_bounds__(kYBlock) k_sweep_y(SweepArgs<T> a, int ncolblk
     if (a.reduce) reduce_cfl(a.st, local);
 }
 
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kYBlock) k_sweep_y(SweepArgs<T> a, int ncolblk)
+{
+    if (!a.st->active) return;
+    const T dt = (T) (a.st->current_dt * a.dt_factor);
+    const int seg = blockIdx.x / ncolblk;
+    const int x = (blockIdx.x - seg * ncolblk) * kYBlock + threadIdx.x;
+    const int y0 = seg * a.seg_len;
+    sweep_y_run<T, FLUX, PROJ, EOS>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched ensemble: persistent blocks over the work items of every domain
+// ---------------------------------------------------------------------------------------
+
+template <class T>
+__device__ __forceinline__ SweepArgs<T> batch_domain_args(const BatchArgs<T>& b, int d, int axis)
+{
+    const BatchDomain& D = b.domains[d];
+    SweepArgs<T> a;
+    a.in_rho = b.in_rho + D.offset;
+    a.in_u = b.in_u + D.offset;
+    a.in_v = b.in_v + D.offset;
+    a.in_E = b.in_E + D.offset;
+    a.out_rho = b.out_rho + D.offset;
+    a.out_u = b.out_u + D.offset;
+    a.out_v = b.out_v + D.offset;
+    a.out_E = b.out_E + D.offset;
+    a.stale_p = b.stale_p + D.offset;
+    a.stale_c = b.stale_c + D.offset;
+    a.st = b.states + d;
+    a.nx = D.nx;
+    a.ny = D.ny;
+    a.row_length = D.row_length;
+    a.index_start = D.index_start;
+    a.stencil_width = b.stencil_width;
+    a.seg_len = 0;
+    a.dx_pow2 = D.dx_pow2[axis];
+    a.reduce = b.reduce;
+    a.gamma = b.gamma;
+    a.dx = D.dx[axis];
+    a.inv_dx = D.inv_dx[axis];
+    a.dt_factor = b.dt_factor;
+    const int lo = axis == 0 ? 0 : 3, hi = axis == 0 ? 1 : 2;   // Left/Right or Bottom/Top
+    a.lo_u = D.bc[lo][0];
+    a.lo_v = D.bc[lo][1];
+    a.hi_u = D.bc[hi][0];
+    a.hi_v = D.bc[hi][1];
+    a.cfl_dx = D.cfl_dx;
+    a.cfl_dy = D.cfl_dy;
+    return a;
+}
+
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kTile) k_batch_x(BatchArgs<T> b, const BatchItemX* __restrict__ items, int nitems)
+{
+    __shared__ SmemX<T> sm;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchItemX item = items[it];
+        const DeviceState* st = b.states + item.domain;
+        if (!st->active) continue;   // the same for the whole block
+        const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
+        sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.x0, sm);
+        __syncthreads();             // the next item reuses the LDS
+    }
+}
+
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kYBlock) k_batch_y(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
+{
+    const int lane = threadIdx.x & 63;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int it = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; it < nitems; it += nwaves) {
+        const BatchItemY item = items[it];
+        const DeviceState* st = b.states + item.domain;
+        if (!st->active) continue;   // the same for the whole wave
+        const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
+        const int x = item.x0 + lane;
+        sweep_y_run<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0, item.y1);
+    }
+}
+
 inline unsigned grid_for(int64_t n, int block, int64_t max_blocks)
 {
     int64_t g = (n + block - 1) / block;
@@ -462,6 +550,45 @@ void launch_proj_dispatch(int axis, const SweepVariant& v, const SweepArgs<T>& a
     else launch_eos_dispatch<T, F, 1>(axis, v.eos, a, s);
 }
 
+template <class T, int F, int P, int E>
+void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, int nitems, int grid, hipStream_t s)
+{
+    if (nitems <= 0) return;
+    if (axis == 0)
+        k_batch_x<T, F, P, E><<<dim3((unsigned) grid), dim3(kTile), 0, s>>>(b, static_cast<const BatchItemX*>(items),
+                                                                             nitems);
+    else
+        k_batch_y<T, F, P, E><<<dim3((unsigned) grid), dim3(kYBlock), 0, s>>>(b, static_cast<const BatchItemY*>(items),
+                                                                               nitems);
+    check_launch(axis == 0 ? "batch_x" : "batch_y");
+}
+
+template <class T, int F, int P, int E>
+int batch_variant_resident(int axis)
+{
+    return axis == 0 ? resident_blocks((const void*) k_batch_x<T, F, P, E>, kTile)
+                     : resident_blocks((const void*) k_batch_y<T, F, P, E>, kYBlock);
+}
+
+// Calls fn.template operator()<F, P, E>() for the variant's compile-time parameters.
+template <class Fn>
+auto with_variant(const SweepVariant& v, Fn&& fn)
+{
+    auto by_eos = [&](auto f, auto p) {
+        return v.eos == 0 ? fn.template operator()<decltype(f)::value, decltype(p)::value, 0>()
+                          : fn.template operator()<decltype(f)::value, decltype(p)::value, 1>();
+    };
+    auto by_proj = [&](auto f) {
+        return v.proj == 2 ? by_eos(f, std::integral_constant<int, 2>{}) : by_eos(f, std::integral_constant<int, 1>{});
+    };
+    switch (v.flux) {
+    case 0: return by_proj(std::integral_constant<int, 0>{});
+    case 1: return by_proj(std::integral_constant<int, 1>{});
+    case 2: return by_proj(std::integral_constant<int, 2>{});
+    default: return by_proj(std::integral_constant<int, 3>{});
+    }
+}
+
 }  // namespace sweep
 
 template <class T>
@@ -473,6 +600,22 @@ void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStr
     case 2: sweep::launch_proj_dispatch<T, 2>(axis, v, a, s); break;
     default: sweep::launch_proj_dispatch<T, 3>(axis, v, a, s); break;
     }
+}
+
+template <class T>
+void launch_batch_sweep(int axis, const SweepVariant& v, const BatchArgs<T>& b, const void* items, int nitems, int grid,
+                        hipStream_t s)
+{
+    sweep::with_variant(v, [&]<int F, int P, int E>() {
+        sweep::launch_batch_variant<T, F, P, E>(axis, b, items, nitems, grid, s);
+        return 0;
+    });
+}
+
+template <class T>
+int batch_resident_blocks(int axis, const SweepVariant& v)
+{
+    return sweep::with_variant(v, [&]<int F, int P, int E>() { return sweep::batch_variant_resident<T, F, P, E>(axis); });
 }
 
 }  // namespace armon

@@ -57,6 +57,28 @@ ARMON_EXPORT void dt_CFL_device(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D
                                 ARMON_CREF(view) umat, ARMON_CREF(view) vmat, ARMON_CREF(view) cmat,
                                 ARMON_CREF(mask_view) domain_mask, double* dt_device);
 
+/* ---- initial states -------------------------------------------------------- */
+
+/* High/low region states of a test case (init_test.cpp:12-48, TestParams of test_cases.h:17-22). */
+typedef struct ArmonTestState {
+    double high_rho, low_rho, high_E, low_E, high_u, low_u, high_v, low_v;
+} ArmonTestState;
+
+/* The reference states of `test` (test_option: the Sedov radius, test_cases.h:118-120). */
+ARMON_EXPORT void armon_reference_test_state(int test, double test_option, ArmonTestState* out);
+
+/* init_test with caller-given high/low states instead of the test's reference ones (used by
+ * the batched ensemble, whose domains perturb their initial states). Same regions, masks and
+ * initial p, c, g, ustar, pstar as init_test; no debug-index mode. */
+ARMON_EXPORT void init_test_states(ARMON_CREF(Range) range, ARMON_CREF(InnerRange2D) inner_range,
+                                   Idx row_length, Idx nb_ghosts, Idx nx, Idx ny, flt_t sx,
+                                   flt_t sy, flt_t ox, flt_t oy, Idx pos_x, Idx pos_y, Idx g_nx, Idx g_ny,
+                                   ARMON_REF(view) x, ARMON_REF(view) y, ARMON_REF(view) rho, ARMON_REF(view) Emat,
+                                   ARMON_REF(view) umat, ARMON_REF(view) vmat, ARMON_REF(mask_view) domain_mask,
+                                   ARMON_REF(view) pmat, ARMON_REF(view) cmat, ARMON_REF(view) gmat,
+                                   ARMON_REF(view) ustar, ARMON_REF(view) pstar,
+                                   Test test, flt_t test_option, ARMON_CREF(ArmonTestState) states);
+
 /* ---- measurement --------------------------------------------------------- */
 
 /* Bandwidth (GB/s, bytes read + written per second) of a plain device copy of `bytes`
