@@ -97,6 +97,16 @@ class ArmonTestState(C.Structure):
                                           "high_u", "low_u", "high_v", "low_v")]
 
 
+class ArmonDomainSpec(C.Structure):
+    _fields_ = [("test", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("max_time", C.c_double),
+                ("state", ArmonTestState)]
+
+
+class ArmonBatchTuning(C.Structure):
+    _fields_ = [("y_run_rows", C.c_int), ("grid_x", C.c_int), ("grid_y", C.c_int), ("items_x", C.c_int),
+                ("items_y", C.c_int), ("ms_x", C.c_double), ("ms_y", C.c_double)]
+
+
 class ArmonKernelTimes(C.Structure):
     _fields_ = [("launches", C.c_int * 4), ("total_ms", C.c_double * 4)]
 
@@ -183,6 +193,26 @@ EXTENSION_SYMBOLS = {
     "armon_solver_conservation": (C.c_int, [C.c_void_p, C.POINTER(_d), C.POINTER(_d)]),
     "armon_solver_stream": (C.c_void_p, [C.c_void_p]),
     "armon_run": (C.c_int, [_PP, C.POINTER(ArmonResult)]),
+    "armon_batch_random_specs": (C.c_int, [C.c_uint64, C.c_int, C.c_int64, C.c_int64, _d,
+                                           C.POINTER(ArmonDomainSpec)]),
+    "armon_batch_create": (C.c_void_p, [_PP, C.c_int, C.POINTER(ArmonDomainSpec)]),
+    "armon_batch_destroy": (None, [C.c_void_p]),
+    "armon_batch_count": (C.c_int, [C.c_void_p]),
+    "armon_batch_cells": (C.c_int64, [C.c_void_p]),
+    "armon_batch_domain_params": (C.c_int, [C.c_void_p, C.c_int, _PP]),
+    "armon_batch_stream": (C.c_void_p, [C.c_void_p]),
+    "armon_batch_init": (C.c_int, [C.c_void_p]),
+    "armon_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(ArmonBatchTuning)]),
+    "armon_batch_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(ArmonBatchTuning)]),
+    "armon_batch_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(ArmonBatchTuning)]),
+    "armon_batch_begin": (C.c_int, [C.c_void_p]),
+    "armon_batch_advance": (C.c_int, [C.c_void_p, C.c_int]),
+    "armon_batch_run": (C.c_int, [C.c_void_p]),
+    "armon_batch_active": (C.c_int, [C.c_void_p]),
+    "armon_batch_finish": (C.c_int, [C.c_void_p]),
+    "armon_batch_domain_result": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ArmonResult)]),
+    "armon_batch_read_field": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_d), C.c_int64]),
+    "armon_batch_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ArmonKernelTimes)]),
 }
 
 _lib = None
@@ -377,3 +407,121 @@ def run(params: ArmonParams, init_fields: Optional[dict] = None):
         s.write(k, v)
     r = s.time_loop()
     return r, s
+
+
+# ---- batched ensemble ------------------------------------------------------------------
+def reference_state(test, test_option: float = 0.0) -> ArmonTestState:
+    """The reference high/low states of a test (init_test.cpp:12-48)."""
+    st = ArmonTestState()
+    load().armon_reference_test_state(TESTS[test] if isinstance(test, str) else int(test), float(test_option),
+                                      C.byref(st))
+    return st
+
+
+def random_specs(seed: int, n: int, min_area: int = 64 * 64, max_area: int = 1024 * 1024,
+                 max_aspect: float = 4.0):
+    """``n`` domain specs drawn from splitmix64(seed) (armon_batch_random_specs)."""
+    arr = (ArmonDomainSpec * n)()
+    _check(load().armon_batch_random_specs(C.c_uint64(seed), int(n), int(min_area), int(max_area),
+                                           float(max_aspect), arr), "armon_batch_random_specs")
+    return arr
+
+
+def batch_params(**kw) -> ArmonParams:
+    """Common parameters of a batch: the reference defaults plus overrides (no test-specific
+    init; cfl 0 means each domain's test default)."""
+    p = ArmonParams()
+    load().armon_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Batch:
+    """Many independent domains advanced together (armon_batch_*)."""
+
+    def __init__(self, common: ArmonParams, specs):
+        lib = load()
+        n = len(specs)
+        if not isinstance(specs, C.Array):
+            arr = (ArmonDomainSpec * n)()
+            for k, sp in enumerate(specs):
+                arr[k] = sp
+            specs = arr
+        self.specs = specs
+        self.h = lib.armon_batch_create(C.byref(common), n, specs)
+        if not self.h:
+            raise ArmonError(f"armon_batch_create failed: {lib.armon_last_error().decode()} {errors()}")
+        self.n = n
+
+    def init(self):
+        _check(load().armon_batch_init(self.h), "armon_batch_init")
+
+    def autotune(self) -> ArmonBatchTuning:
+        t = ArmonBatchTuning()
+        _check(load().armon_batch_autotune(self.h, C.byref(t)), "armon_batch_autotune")
+        return t
+
+    def tuning(self) -> ArmonBatchTuning:
+        t = ArmonBatchTuning()
+        load().armon_batch_get_tuning(self.h, C.byref(t))
+        return t
+
+    def set_tuning(self, y_run_rows: int, grid_x: int = 0, grid_y: int = 0):
+        t = ArmonBatchTuning(y_run_rows=y_run_rows, grid_x=grid_x, grid_y=grid_y)
+        _check(load().armon_batch_set_tuning(self.h, C.byref(t)), "armon_batch_set_tuning")
+
+    def begin(self):
+        _check(load().armon_batch_begin(self.h), "armon_batch_begin")
+
+    def advance(self, n: int):
+        _check(load().armon_batch_advance(self.h, int(n)), "armon_batch_advance")
+
+    def run(self):
+        _check(load().armon_batch_run(self.h), "armon_batch_run")
+
+    def active(self) -> int:
+        return load().armon_batch_active(self.h)
+
+    def finish(self) -> int:
+        return load().armon_batch_finish(self.h)
+
+    def result(self, d: int) -> ArmonResult:
+        r = ArmonResult()
+        _check(load().armon_batch_domain_result(self.h, int(d), C.byref(r)), "armon_batch_domain_result")
+        return r
+
+    def params(self, d: int) -> ArmonParams:
+        p = ArmonParams()
+        _check(load().armon_batch_domain_params(self.h, int(d), C.byref(p)), "armon_batch_domain_params")
+        return p
+
+    def cells(self) -> int:
+        return load().armon_batch_cells(self.h)
+
+    def read(self, d: int, field) -> np.ndarray:
+        f = FIELD[field] if isinstance(field, str) else int(field)
+        n = self.params(d).nb_cells
+        out = np.empty(n, dtype=np.float64)
+        _check(load().armon_batch_read_field(self.h, int(d), f, out.ctypes.data_as(C.POINTER(C.c_double)), n),
+               "armon_batch_read_field")
+        return out
+
+    def kernel_times(self, n: int) -> ArmonKernelTimes:
+        t = ArmonKernelTimes()
+        _check(load().armon_batch_kernel_times(self.h, int(n), C.byref(t)), "armon_batch_kernel_times")
+        return t
+
+    def stream(self):
+        return load().armon_batch_stream(self.h)
+
+    def close(self):
+        if self.h:
+            load().armon_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -28,6 +28,8 @@ inline Error hip_failure(const std::string& m) { return Error("hip_error", m); }
 // Reports `e` through the handler; also records it for armon_last_error().
 void report(const char* kernel, const Error& e);
 void report_std(const char* kernel, const std::exception& e);
+// Errors reported so far in this process (a caller compares it before and after a call).
+uint64_t report_count();
 
 inline void hip_check(hipError_t err, const char* what)
 {

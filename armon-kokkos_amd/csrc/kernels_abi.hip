@@ -35,8 +35,13 @@ static thread_local std::string g_last_error;
 
 namespace armon {
 
+static uint64_t g_report_count = 0;
+
+uint64_t report_count() { return g_report_count; }
+
 void report(const char* kernel, const Error& e)
 {
+    g_report_count++;
     std::string msg = "(" + e.type + ") " + e.what();
     g_last_error = std::string(kernel) + ": " + msg;
     raise_exception_handler(kernel, msg.c_str());

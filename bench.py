@@ -30,9 +30,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles):
+def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles, graph_chunk):
     p = armon.make_params("Sedov", nx=nx, ny=ny, max_cycles=10**9, max_time=1e9, verbose=5, engine=1,
-                          precision=precision, graph_chunk=16, **WORKLOAD)
+                          precision=precision, graph_chunk=graph_chunk, **WORKLOAD)
     s = armon.Solver(p)
     s.init_test()
     lib = armon.load()
@@ -81,7 +81,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=8192)
-    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--cpu-cycles", type=int, default=40)
+    ap.add_argument("--graph-chunk", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
@@ -101,7 +102,7 @@ def main():
     copy_gbps = armon.load().armon_copy_bandwidth(1 << 31, 20)
     log(f"[bench] device copy bandwidth (2 GiB buffers): {copy_gbps:.0f} GB/s")
     log(f"[bench] FP64 fused, Sedov {n}x{n}, warmup {args.warmup}, steps {args.steps}")
-    el64, kt64, rho64 = measure_gpu(armon, n, n, args.steps, args.warmup, 0, 10)
+    el64, kt64, rho64 = measure_gpu(armon, n, n, args.steps, args.warmup, 0, 10, args.graph_chunk)
     rate64 = cells * args.steps / el64
     log(f"[bench] FP64: {rate64 / 1e9:.3f} G cell-updates/s, {el64 / args.steps * 1e3:.3f} ms/cycle")
 
@@ -116,7 +117,7 @@ def main():
     fp32 = None
     if not args.no_fp32:
         log("[bench] FP32-mixed fused")
-        el32, kt32, rho32 = measure_gpu(armon, n, n, args.steps, args.warmup, 1, 10)
+        el32, kt32, rho32 = measure_gpu(armon, n, n, args.steps, args.warmup, 1, 10, args.graph_chunk)
         rate32 = cells * args.steps / el32
         real = np.zeros_like(rho64, dtype=bool)
         g_ = WORKLOAD["nb_ghosts"]
@@ -153,7 +154,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (reference Sedov initial state)",
         "config": {"workload": f"Sedov {n}x{n}, 4 ghosts, GAD+minmod, Euler 2nd order remap, Sequential "
-                               f"splitting, CFL 0.7, fused engine, hipGraph of 16 cycles",
+                               f"splitting, CFL 0.7, fused engine, hipGraph of {args.graph_chunk} cycles",
                    "cells": cells, "engine": "fused"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None, "kernel": names[dom],

@@ -210,6 +210,76 @@ ARMON_EXPORT void* armon_solver_stream(ArmonSolver* s);
 /* armon_2D.cpp:389-431: allocate, init, loop, optional output. */
 ARMON_EXPORT int armon_run(ArmonParams* p, ArmonResult* out);
 
+/* ---- batched ensemble ------------------------------------------------------
+ * Many independent domains of different sizes advanced together: every sweep of a cycle
+ * is ONE persistent launch over the work items (row tiles for X, column runs for Y) of all
+ * domains, each domain with its own dt, time, cycle count and end time, all on the device;
+ * cycles are replayed from a captured hipGraph. Each domain follows exactly the reference
+ * time loop (armon_2D.cpp:327-386) on its own parameters, so a domain's result is the
+ * single-domain result. This has no counterpart in the reference, which runs one domain
+ * per process. */
+
+typedef struct ArmonDomainSpec {
+    int test;                 /* Sod, Sod_y, Sod_circ or Sedov (perfect gas); or Bizarrium for every domain */
+    int nx, ny;               /* real cells */
+    double max_time;          /* end time; <= 0: the test's default */
+    ArmonTestState state;     /* high/low region states (armon_reference_test_state for the reference ones) */
+} ArmonDomainSpec;
+
+/* `n` domains drawn from splitmix64(seed): area log-uniform in [min_area, max_area] cells,
+ * aspect ratio log-uniform in [1, max_aspect] with a random orientation, test uniform over
+ * {Sod, Sod_circ, Sedov}, and the high/low densities and pressures of the reference states
+ * each multiplied by a factor uniform in [0.8, 1.2] (E scaled so that p follows). Sides are
+ * at least 16 cells. Returns 0, or 1 on invalid arguments. */
+ARMON_EXPORT int armon_batch_random_specs(uint64_t seed, int n, int64_t min_area, int64_t max_area,
+                                          double max_aspect, ArmonDomainSpec* out);
+
+/* Launch configuration of the batched sweeps, chosen by armon_batch_autotune. */
+typedef struct ArmonBatchTuning {
+    int y_run_rows;           /* rows a wave marches per Y work item (64 columns wide) */
+    int grid_x, grid_y;       /* blocks of the X and Y launches (persistent when below the item count) */
+    int items_x, items_y;     /* work items per X / Y sweep (outputs) */
+    double ms_x, ms_y;        /* measured time per X / Y sweep of this configuration (autotune output) */
+} ArmonBatchTuning;
+
+typedef struct ArmonBatch ArmonBatch;
+
+/* `common` (armon_params_default plus overrides) gives the scheme, limiter, projection,
+ * splitting, ghosts, stencil width, Dt/cst_dt, dt_on_even_cycles, max_cycles, precision and
+ * graph_chunk of every domain, and its cfl when positive (else each test's default);
+ * `specs` the test, size, states and end time of each domain. */
+ARMON_EXPORT ArmonBatch* armon_batch_create(const ArmonParams* common, int n, const ArmonDomainSpec* specs);
+ARMON_EXPORT void armon_batch_destroy(ArmonBatch* b);
+ARMON_EXPORT int armon_batch_count(const ArmonBatch* b);
+ARMON_EXPORT int64_t armon_batch_cells(const ArmonBatch* b);            /* real cells of all domains */
+ARMON_EXPORT int armon_batch_domain_params(const ArmonBatch* b, int d, ArmonParams* out);
+ARMON_EXPORT void* armon_batch_stream(ArmonBatch* b);
+/* init_test_states of every domain (armon_2D.cpp:112-127). */
+ARMON_EXPORT int armon_batch_init(ArmonBatch* b);
+/* Times the sweeps over the initial state for the candidate configurations (Y run length,
+ * persistent or one-block-per-item grids) and keeps the fastest. After init, before begin. */
+ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out);
+/* Sets the configuration by hand (grid_x/grid_y <= 0: one block per item); before begin. */
+ARMON_EXPORT int armon_batch_set_tuning(ArmonBatch* b, const ArmonBatchTuning* t);
+ARMON_EXPORT int armon_batch_get_tuning(const ArmonBatch* b, ArmonBatchTuning* out);
+ARMON_EXPORT int armon_batch_begin(ArmonBatch* b);
+/* Enqueues up to n more cycles of every still-running domain, without synchronising. */
+ARMON_EXPORT int armon_batch_advance(ArmonBatch* b, int n);
+/* Advances until every domain has reached its end time or max_cycles (or failed). */
+ARMON_EXPORT int armon_batch_run(ArmonBatch* b);
+/* Number of domains still running (synchronises). */
+ARMON_EXPORT int armon_batch_active(ArmonBatch* b);
+/* Puts every domain's final state into its rho/umat/vmat/Emat and the EOS of its last
+ * sweep's input into pmat/cmat/gmat, as the single-domain solver does. Returns 0, or 1 if
+ * some domain stopped on an invalid time step. */
+ARMON_EXPORT int armon_batch_finish(ArmonBatch* b);
+/* cycles, time, current_dt, final_dt and status of domain d (after finish). */
+ARMON_EXPORT int armon_batch_domain_result(ArmonBatch* b, int d, ArmonResult* out);
+/* Copies field `field` of domain d (its nb_cells values) to the host. */
+ARMON_EXPORT int armon_batch_read_field(ArmonBatch* b, int d, int field, double* host, int64_t n);
+/* Runs n cycles launch by launch, timing every kernel with HIP events. After begin. */
+ARMON_EXPORT int armon_batch_kernel_times(ArmonBatch* b, int n, ArmonKernelTimes* out);
+
 #ifdef __cplusplus
 }
 #endif

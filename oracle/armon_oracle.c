@@ -486,9 +486,7 @@ void o_write_border_array(const ORange* range, const OInnerRange2D* ir, o_idx si
 }
 
 /* init_test.cpp:12-48 and test_cases.h:16-121: per-test initial states */
-typedef struct { double high_rho, low_rho, high_E, low_E, high_u, low_u, high_v, low_v; } OTestParams;
-
-static void test_init_params(int test, double r, OTestParams* t)
+void o_reference_test_state(int test, double r, OTestState* t)
 {
     memset(t, 0, sizeof(*t));
     switch (test) {
@@ -524,16 +522,18 @@ static inline int test_region_high(int test, double r, double x, double y)
 
 /* init_test.cpp:51-104 (template body) and :107-143 (dispatch). Returns -1
  * for an invalid test index (the reference throws std::out_of_range). */
-int o_init_test(const ORange* range, const OInnerRange2D* ir,
-                o_idx row_length, o_idx nb_ghosts, o_idx nx, o_idx ny, double sx, double sy,
-                double ox, double oy, o_idx pos_x, o_idx pos_y, o_idx g_nx, o_idx g_ny,
-                double* x, double* y, double* rho, double* Emat, double* umat, double* vmat,
-                double* domain_mask, double* pmat, double* cmat, double* gmat,
-                double* ustar, double* pstar, int test, bool debug_indexes, double test_option)
+/* init_test.cpp:51-94 with the high/low states given by the caller (o_init_test passes the
+ * reference's states of `test`; the batched ensemble passes perturbed ones). */
+int o_init_test_states(const ORange* range, const OInnerRange2D* ir,
+                       o_idx row_length, o_idx nb_ghosts, o_idx nx, o_idx ny, double sx, double sy,
+                       double ox, double oy, o_idx pos_x, o_idx pos_y, o_idx g_nx, o_idx g_ny,
+                       double* x, double* y, double* rho, double* Emat, double* umat, double* vmat,
+                       double* domain_mask, double* pmat, double* cmat, double* gmat,
+                       double* ustar, double* pstar, int test, bool debug_indexes, double test_option,
+                       const OTestState* states)
 {
     if (test < O_TEST_SOD || test > O_TEST_SEDOV) return -1;
-    OTestParams tp;
-    test_init_params(test, test_option, &tp);
+    const OTestState tp = *states;
     const OInnerRange2D r = *ir;
 
     PARALLEL_FOR
@@ -571,6 +571,21 @@ int o_init_test(const ORange* range, const OInnerRange2D* ir,
         pstar[i] = 0;
     }
     return 0;
+}
+
+int o_init_test(const ORange* range, const OInnerRange2D* ir,
+                o_idx row_length, o_idx nb_ghosts, o_idx nx, o_idx ny, double sx, double sy,
+                double ox, double oy, o_idx pos_x, o_idx pos_y, o_idx g_nx, o_idx g_ny,
+                double* x, double* y, double* rho, double* Emat, double* umat, double* vmat,
+                double* domain_mask, double* pmat, double* cmat, double* gmat,
+                double* ustar, double* pstar, int test, bool debug_indexes, double test_option)
+{
+    if (test < O_TEST_SOD || test > O_TEST_SEDOV) return -1;
+    OTestState tp;
+    o_reference_test_state(test, test_option, &tp);
+    return o_init_test_states(range, ir, row_length, nb_ghosts, nx, ny, sx, sy, ox, oy, pos_x, pos_y, g_nx, g_ny,
+                              x, y, rho, Emat, umat, vmat, domain_mask, pmat, cmat, gmat, ustar, pstar, test,
+                              debug_indexes, test_option, &tp);
 }
 
 /* reductions.cpp:8-24. Min is order independent, so threads do not change the result.
@@ -930,21 +945,38 @@ void o_cell_update_params(const OParams* p, OData* d)
     o_cell_update(&r, &ir, p->s, p->dx, p->cycle_dt, d->ustar, d->pstar, d->rho, u, d->Emat);
 }
 
+/* test_cases.h:117-119: only Sedov has a test option (armon_2D.cpp:116-118) */
+double o_test_option(const OParams* p)
+{
+    double dx = p->domain_size[0] / (double) p->nx;
+    double dy = p->domain_size[1] / (double) p->ny;
+    return p->test == O_TEST_SEDOV ? hypot(dx, dy) / M_SQRT2 : 0.;
+}
+
 /* armon_2D.cpp:112-127 */
 void o_init_test_params(const OParams* p, OData* d, bool debug_indexes)
 {
     ODomainRange dr = o_complete_domain(p);
     ORange r; OInnerRange2D ir;
     o_dr_iter2d(&dr, &r, &ir);
-    double dx = p->domain_size[0] / (double) p->nx;
-    double dy = p->domain_size[1] / (double) p->ny;
-    /* test_cases.h:117-119: only Sedov has a test option */
-    double test_option = p->test == O_TEST_SEDOV ? hypot(dx, dy) / M_SQRT2 : 0.;
     o_init_test(&r, &ir, p->row_length, p->nb_ghosts, p->nx, p->ny,
                 p->domain_size[0], p->domain_size[1], p->domain_origin[0], p->domain_origin[1],
                 0, 0, p->nx, p->ny,
                 d->x, d->y, d->rho, d->Emat, d->umat, d->vmat, d->domain_mask,
-                d->pmat, d->cmat, d->gmat, d->ustar, d->pstar, p->test, debug_indexes, test_option);
+                d->pmat, d->cmat, d->gmat, d->ustar, d->pstar, p->test, debug_indexes, o_test_option(p));
+}
+
+/* o_init_test_params with caller-given high/low states (a batched-ensemble domain). */
+void o_init_test_params_states(const OParams* p, OData* d, const OTestState* states)
+{
+    ODomainRange dr = o_complete_domain(p);
+    ORange r; OInnerRange2D ir;
+    o_dr_iter2d(&dr, &r, &ir);
+    o_init_test_states(&r, &ir, p->row_length, p->nb_ghosts, p->nx, p->ny,
+                       p->domain_size[0], p->domain_size[1], p->domain_origin[0], p->domain_origin[1],
+                       0, 0, p->nx, p->ny,
+                       d->x, d->y, d->rho, d->Emat, d->umat, d->vmat, d->domain_mask,
+                       d->pmat, d->cmat, d->gmat, d->ustar, d->pstar, p->test, false, o_test_option(p), states);
 }
 
 /* armon_2D.cpp:130-140 */

@@ -117,6 +117,16 @@ void o_write_border_array(const ORange* range, const OInnerRange2D* ir, o_idx si
                           double* rho, double* umat, double* vmat, double* pmat,
                           double* cmat, double* gmat, double* Emat,
                           const double* value_array);
+/* High/low region states of a test (init_test.cpp:12-48). */
+typedef struct { double high_rho, low_rho, high_E, low_E, high_u, low_u, high_v, low_v; } OTestState;
+void o_reference_test_state(int test, double test_option, OTestState* out);
+int  o_init_test_states(const ORange* range, const OInnerRange2D* ir,
+                        o_idx row_length, o_idx nb_ghosts, o_idx nx, o_idx ny, double sx, double sy,
+                        double ox, double oy, o_idx pos_x, o_idx pos_y, o_idx g_nx, o_idx g_ny,
+                        double* x, double* y, double* rho, double* Emat, double* umat, double* vmat,
+                        double* domain_mask, double* pmat, double* cmat, double* gmat,
+                        double* ustar, double* pstar, int test, bool debug_indexes, double test_option,
+                        const OTestState* states);
 int  o_init_test(const ORange* range, const OInnerRange2D* ir,
                  o_idx row_length, o_idx nb_ghosts, o_idx nx, o_idx ny, double sx, double sy,
                  double ox, double oy, o_idx pos_x, o_idx pos_y, o_idx g_nx, o_idx g_ny,
@@ -180,7 +190,9 @@ int  o_data_alloc(OData* d, int64_t nb_cells);        /* 64-byte aligned */
 void o_data_free(OData* d);
 
 /* ---- driver (armon_2D.cpp) ---- */
+double o_test_option(const OParams* p);                                  /* armon_2D.cpp:116-118 */
 void   o_init_test_params(const OParams* p, OData* d, bool debug_indexes); /* armon_2D.cpp:112-127 */
+void   o_init_test_params_states(const OParams* p, OData* d, const OTestState* states);
 void   o_update_EOS(const OParams* p, OData* d);                          /* armon_2D.cpp:85-101 */
 void   o_boundary_conditions_all(const OParams* p, OData* d);             /* armon_2D.cpp:130-151 */
 void   o_numerical_fluxes(const OParams* p, OData* d);                    /* armon_2D.cpp:70-82 */

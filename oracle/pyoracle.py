@@ -76,6 +76,12 @@ _P = C.POINTER(C.c_double)
 _libs: dict = {}
 
 
+class OTestState(C.Structure):
+    """High/low region states of a test case (init_test.cpp:12-48)."""
+    _fields_ = [(n, C.c_double) for n in ("high_rho", "low_rho", "high_E", "low_E",
+                                          "high_u", "low_u", "high_v", "low_v")]
+
+
 def _proto(lib):
     i64, d = C.c_int64, C.c_double
     R, I1, I2 = C.POINTER(ORange), C.POINTER(OInnerRange1D), C.POINTER(OInnerRange2D)
@@ -98,6 +104,10 @@ def _proto(lib):
         "o_write_border_array": (None, [R, I2, i64, i64, _P, _P, _P, _P, _P, _P, _P, _P]),
         "o_init_test": (C.c_int, [R, I2, i64, i64, i64, i64, d, d, d, d, i64, i64, i64, i64,
                                   _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_bool, d]),
+        "o_reference_test_state": (None, [C.c_int, d, C.POINTER(OTestState)]),
+        "o_init_test_states": (C.c_int, [R, I2, i64, i64, i64, i64, d, d, d, d, i64, i64, i64, i64,
+                                         _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_bool, d,
+                                         C.POINTER(OTestState)]),
         "o_dt_CFL": (d, [R, I1, d, d, _P, _P, _P, _P]),
         "o_conservation_vars": (None, [R, I1, d, _P, _P, _P, _P, _P]),
         "o_params_defaults": (None, [C.POINTER(OParams)]),
@@ -124,7 +134,9 @@ def _proto(lib):
         "o_dr_iter2d": (None, [C.POINTER(ODomainRange), R, I2]),
         "o_data_alloc": (C.c_int, [C.POINTER(OData), C.c_int64]),
         "o_data_free": (None, [C.POINTER(OData)]),
+        "o_test_option": (d, [C.POINTER(OParams)]),
         "o_init_test_params": (None, [C.POINTER(OParams), C.POINTER(OData), C.c_bool]),
+        "o_init_test_params_states": (None, [C.POINTER(OParams), C.POINTER(OData), C.POINTER(OTestState)]),
         "o_update_EOS": (None, [C.POINTER(OParams), C.POINTER(OData)]),
         "o_boundary_conditions_all": (None, [C.POINTER(OParams), C.POINTER(OData)]),
         "o_numerical_fluxes": (None, [C.POINTER(OParams), C.POINTER(OData)]),
@@ -211,10 +223,14 @@ class Data:
             pass
 
 
-def run(lib, p: OParams, debug_indexes: bool = False):
-    """init_test + time_loop (armon_2D.cpp:389-407). Returns (status, data, cycles, final_dt, loop_time)."""
+def run(lib, p: OParams, debug_indexes: bool = False, states: "OTestState | None" = None):
+    """init_test + time_loop (armon_2D.cpp:389-407). Returns (status, data, cycles, final_dt, loop_time).
+    ``states`` replaces the test's reference high/low states (a batched-ensemble domain)."""
     d = Data(lib, p.nb_cells)
-    lib.o_init_test_params(C.byref(p), C.byref(d.c), bool(debug_indexes))
+    if states is not None:
+        lib.o_init_test_params_states(C.byref(p), C.byref(d.c), C.byref(states))
+    else:
+        lib.o_init_test_params(C.byref(p), C.byref(d.c), bool(debug_indexes))
     loop_time, final_dt, cycles = C.c_double(), C.c_double(), C.c_int()
     st = lib.o_time_loop(C.byref(p), C.byref(d.c), C.byref(loop_time), C.byref(final_dt), C.byref(cycles))
     return st, d, cycles.value, final_dt.value, loop_time.value
