@@ -82,6 +82,7 @@ class ArmonParams(C.Structure):
         ("write_output", C.c_int), ("write_ghosts", C.c_int), ("write_throughput", C.c_int),
         ("output_precision", C.c_int), ("output_file", C.c_char_p),
         ("engine", C.c_int), ("precision", C.c_int), ("graph_chunk", C.c_int),
+        ("compare", C.c_int), ("comparison_tolerance", C.c_double),
     ]
 
 

@@ -11,12 +11,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -143,6 +145,7 @@ struct ArmonSolver {
     std::chrono::steady_clock::time_point t_begin;
     int error_count_at_begin = 0;
     bool dt_error = false;           // an invalid time step stopped the loop (armon_2D.cpp:238-242)
+    bool compare_stop = false;       // a per-step comparison found a difference
 
     ~ArmonSolver();
 };
@@ -166,6 +169,8 @@ struct StreamScope {
 };
 
 view& V(ArmonSolver* s, int f) { return s->v[f]; }
+
+bool step_checkpoint(ArmonSolver* s, const char* label);
 
 // ---- staged engine (armon_2D.cpp:70-209) --------------------------------------------
 
@@ -304,6 +309,7 @@ bool staged_solver_cycle(ArmonSolver* s)
 {
     ArmonParams& p = s->p;
     if (!staged_time_step(s)) return false;
+    if (step_checkpoint(s, "time_step")) return false;
     int axes[3];
     double factors[3];
     const int n = armon_split_axes(&p, axes, factors);
@@ -311,10 +317,15 @@ bool staged_solver_cycle(ArmonSolver* s)
         armon_update_axis(&p, axes[k]);
         p.cycle_dt = p.current_cycle_dt * factors[k];
         staged_update_EOS(s);
+        if (step_checkpoint(s, "EOS")) return false;
         staged_boundary_conditions(s);
+        if (step_checkpoint(s, "boundary_conditions")) return false;
         staged_numerical_fluxes(s);
+        if (step_checkpoint(s, "numerical_fluxes")) return false;
         staged_cell_update(s);
+        if (step_checkpoint(s, "cell_update")) return false;
         staged_projection_remap(s);
+        if (step_checkpoint(s, "projection_remap")) return false;
     }
     return true;
 }
@@ -659,6 +670,8 @@ ARMON_EXPORT void armon_params_default(ArmonParams* p)
     p->engine = ARMON_ENGINE_FUSED;
     p->precision = ARMON_PREC_FP64;
     p->graph_chunk = 16;
+    p->compare = 0;
+    p->comparison_tolerance = 1e-13;
 }
 
 // parameters.cpp:11-18
@@ -724,6 +737,14 @@ ARMON_EXPORT int armon_params_check(const ArmonParams* p)
     }
     if (p->engine != ARMON_ENGINE_STAGED && p->engine != ARMON_ENGINE_FUSED) {
         fprintf(stderr, "ERROR: unknown engine %d\n", p->engine);
+        return 0;
+    }
+    if (p->compare && p->engine != ARMON_ENGINE_STAGED) {
+        fprintf(stderr, "ERROR: the per-step comparison needs the staged engine\n");
+        return 0;
+    }
+    if (p->compare && p->output_file == nullptr) {
+        fprintf(stderr, "ERROR: the per-step comparison needs an output file name (the step files' prefix)\n");
         return 0;
     }
     if (p->engine == ARMON_ENGINE_STAGED && p->precision != ARMON_PREC_FP64) {
@@ -997,8 +1018,10 @@ ARMON_EXPORT int armon_solver_begin(ArmonSolver* s)
         p.current_cycle_dt = 0.;
         p.next_cycle_dt = 0.;
         s->t_begin = std::chrono::steady_clock::now();
+        s->compare_stop = false;
         armon_update_axis(&p, ARMON_AXIS_X);
         staged_update_EOS(s);
+        step_checkpoint(s, "EOS_init");
         if (p.verbose <= 1) conservation(s, &s->initial_mass, &s->initial_energy);
         if (p.engine == ARMON_ENGINE_FUSED) fused_begin(s);
         s->begun = true;
@@ -1019,6 +1042,7 @@ ARMON_EXPORT int armon_solver_advance(ArmonSolver* s, int n)
             fused_advance(s, n);
             return 0;
         }
+        if (s->compare_stop) return 1;
         for (int k = 0; k < n && p.time < p.max_time && p.cycle < p.max_cycles; k++) {
             const int e0 = g_error_count;
             if (!staged_solver_cycle(s) || g_error_count != e0) return 1;
@@ -1071,6 +1095,7 @@ ARMON_EXPORT int armon_solver_finish(ArmonSolver* s, ArmonResult* out)
         } else {
             hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
             if (s->dt_error) status = 1;
+            else if (s->compare_stop) status = 3;
         }
         const double loop_time =
                 std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_begin).count();
@@ -1201,16 +1226,35 @@ ARMON_EXPORT int armon_solver_conservation(ArmonSolver* s, double* mass, double*
 
 namespace {
 
-void write_output(ArmonSolver* s, const char* file_name)
+// The main variables of the output and comparison files (data.h:120-130).
+const int kMainVars[6] = {ARMON_FIELD_X, ARMON_FIELD_Y, ARMON_FIELD_RHO, ARMON_FIELD_UMAT, ARMON_FIELD_VMAT,
+                          ARMON_FIELD_PMAT};
+using HostVars = std::vector<std::vector<double>>;
+
+HostVars read_main_vars(ArmonSolver* s)
 {
-    const ArmonParams& p = s->p;
-    const int fields[6] = {ARMON_FIELD_X, ARMON_FIELD_Y, ARMON_FIELD_RHO, ARMON_FIELD_UMAT, ARMON_FIELD_VMAT,
-                           ARMON_FIELD_PMAT};
-    std::vector<std::vector<double>> h(6, std::vector<double>((size_t) s->n));
-    for (int f = 0; f < 6; f++) armon_solver_read_field(s, fields[f], h[f].data(), s->n);
+    HostVars h(6, std::vector<double>((size_t) s->n));
+    for (int f = 0; f < 6; f++)
+        if (armon_solver_read_field(s, kMainVars[f], h[f].data(), s->n)) throw hip_failure("reading a field failed");
+    return h;
+}
+
+// Cells of the files: the real domain, plus the ghost layers with write_ghosts (io.cpp:12-15).
+struct FileCells {
+    int i_deb, i_fin, j_deb, j_fin;
+    explicit FileCells(const ArmonParams& p)
+        : i_deb(p.write_ghosts ? -p.nb_ghosts : 0), i_fin(p.nx + (p.write_ghosts ? p.nb_ghosts : 0)),
+          j_deb(p.write_ghosts ? -p.nb_ghosts : 0), j_fin(p.ny + (p.write_ghosts ? p.nb_ghosts : 0))
+    {
+    }
+};
+
+// io.cpp:10-44
+void write_output_host(const ArmonParams& p, const HostVars& h, const char* file_name)
+{
     std::ofstream file(file_name);
-    const int j_deb = p.write_ghosts ? -p.nb_ghosts : 0, j_fin = p.ny + (p.write_ghosts ? p.nb_ghosts : 0);
-    const int i_deb = p.write_ghosts ? -p.nb_ghosts : 0, i_fin = p.nx + (p.write_ghosts ? p.nb_ghosts : 0);
+    const FileCells c(p);
+    const int j_deb = c.j_deb, j_fin = c.j_fin, i_deb = c.i_deb, i_fin = c.i_fin;
     file << std::scientific;
     file.precision(p.output_precision);
     const int width = p.output_precision + 7;
@@ -1226,6 +1270,120 @@ void write_output(ArmonSolver* s, const char* file_name)
     if (p.verbose < 2) printf("Wrote to file: %s\n", file_name);
 }
 
+void write_output(ArmonSolver* s, const char* file_name) { write_output_host(s->p, read_main_vars(s), file_name); }
+
+// io.h:14-17
+bool is_approx(double a, double b, double tol) { return std::abs(a - b) <= tol * std::max(std::abs(a), std::abs(b)); }
+
+// io.cpp:55-74: values separated by ',' and whitespace, one cell per line, in file order.
+void load_data(const ArmonParams& p, HostVars& d, std::istream& file)
+{
+    const FileCells c(p);
+    for (int j = c.j_deb; j < c.j_fin; j++)
+        for (int i = c.i_deb; i < c.i_fin; i++) {
+            const int64_t idx = armon_index_1D(&p, i, j);
+            file >> d[0][idx];
+            for (int f = 1; f < 6; f++) {
+                file.ignore(1, ',');
+                file >> d[f][idx];
+            }
+        }
+}
+
+// io.cpp:77-132: number of cells differing beyond the tolerance, reported row by row.
+int compare_with_reference(const ArmonParams& p, const HostVars& ref, const HostVars& d)
+{
+    static const char* names[6] = {"x", "y", "rho", "umat", "vmat", "pmat"};
+    const FileCells c(p);
+    int total = 0;
+    for (int j = c.j_deb; j < c.j_fin; j++) {
+        const int64_t row_deb = armon_index_1D(&p, c.i_deb, j);
+        const int64_t row_fin = armon_index_1D(&p, c.i_fin, j);
+        for (int f = 0; f < 6; f++) {
+            int count = 0;
+            int64_t diff_start = row_fin, diff_end = row_deb;
+            double max_diff = 0;
+            long max_ulp = 0;
+            for (int64_t idx = row_deb; idx < row_fin; idx++) {
+                const double rv = ref[f][idx], v = d[f][idx];
+                if (is_approx(rv, v, p.comparison_tolerance)) continue;
+                count++;
+                diff_start = std::min(idx, diff_start);
+                diff_end = std::max(idx, diff_end);
+                const double diff = std::abs(rv - v);
+                if (diff > max_diff) {
+                    max_diff = diff;
+                    max_ulp = (long) (max_diff / std::abs(rv * std::numeric_limits<double>::epsilon()));
+                }
+            }
+            total += count;
+            if (count > 0)
+                printf("Row %3d has %3d differences in '%4s' from %3lld to %3lld, max diff: %.15g (%ld ulp)\n", j + 1,
+                       count, names[f], (long long) (diff_start - row_deb + 1), (long long) (diff_end - row_deb + 1),
+                       max_diff, max_ulp);
+        }
+    }
+    return total;
+}
+
+// io.cpp:135-146
+bool compare_with_file(const ArmonParams& p, const HostVars& d, const std::string& name)
+{
+    std::ifstream file(name);
+    file.exceptions(std::ios::failbit | std::istream::badbit);
+    HostVars ref(6, std::vector<double>(d[0].size(), 0.0));
+    load_data(p, ref, file);
+    return compare_with_reference(p, ref, d) > 0;
+}
+
+// io.cpp:149-171
+bool compare_time_step(const ArmonParams& p, const std::string& name)
+{
+    std::ifstream file(name);
+    file.exceptions(std::ios::failbit | std::istream::badbit);
+    double ref_dt;
+    file >> ref_dt;
+    const bool different = !is_approx(ref_dt, p.current_cycle_dt, p.comparison_tolerance);
+    if (different) {
+        const double diff = ref_dt - p.current_cycle_dt;
+        printf("Time step difference: ref \xce\x94t = %.15g, \xce\x94t = %.15g, diff = %.15g (%ld ulp)\n", ref_dt,
+               p.current_cycle_dt, diff, (long) (diff / std::numeric_limits<double>::epsilon()));
+    }
+    return different;
+}
+
+}  // namespace
+
+namespace {
+
+// armon_2D.cpp:255-284: with `compare`, the state after each step is compared with the file
+// <output_file>_<cycle>_<step>_<axis>; on a difference the state is written to <file>_diff
+// and the run stops there.
+bool step_checkpoint(ArmonSolver* s, const char* label)
+{
+    const ArmonParams& p = s->p;
+    if (!p.compare) return false;
+    hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    char buf[100];
+    snprintf(buf, sizeof buf, "_%03d_%s", p.cycle, label);
+    const std::string name = std::string(p.output_file) + buf + "_" + (p.current_axis == ARMON_AXIS_X ? "X" : "Y");
+    HostVars h = read_main_vars(s);
+    bool different;
+    try {
+        different = strcmp(label, "time_step") == 0 ? compare_time_step(p, name) : compare_with_file(p, h, name);
+    } catch (const std::exception& e) {
+        fprintf(stderr, "Error while comparing with file '%s': %s\n", name.c_str(), e.what());
+        different = true;
+    }
+    if (different) {
+        const std::string diff = name + "_diff";
+        write_output_host(p, h, diff.c_str());
+        printf("Difference file written to %s\n", diff.c_str());
+        s->compare_stop = true;
+    }
+    return different;
+}
+
 }  // namespace
 
 extern "C" ARMON_EXPORT int armon_run(ArmonParams* p, ArmonResult* out)
@@ -1235,6 +1393,19 @@ extern "C" ARMON_EXPORT int armon_run(ArmonParams* p, ArmonResult* out)
     ArmonSolver* s = armon_solver_create(p);
     if (!s) return 2;
     int status = armon_solver_init_test(s, false);
+    if (status == 0) {
+        try {
+            if (step_checkpoint(s, "init_test")) status = 3;   // armon_2D.cpp:401
+        } catch (const std::exception& e) {
+            record_error("armon_run", e);
+            status = 2;
+        }
+        if (status != 0) {
+            out->status = status;
+            armon_solver_destroy(s);
+            return status;
+        }
+    }
     if (status == 0) status = armon_solver_time_loop(s, out);
     if (status == 0 && p->verbose < 4) {
         printf("\n");

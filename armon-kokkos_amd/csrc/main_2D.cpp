@@ -32,6 +32,9 @@ Options:
     --precision D           Number of digits of precision printed to the output file for each value.
     --write-throughput 0-1  Enable writing the cell throughput to a separate file (in Mega cells/sec)
     --splitting <method>    Axis splitting method: 'Sequential', 'SequentialSym', 'Strang', 'X_only', 'Y_only'
+    --compare 0-1           Compare each step of the solver with reference files <output>_<cycle>_<step>_<axis>
+                            (staged engine)
+    --comparison-tolerance T  Relative tolerance of the comparison (default 1e-13)
     --verbose 0-3           Verbosity (0: high, 3: low)
     --engine <engine>       'fused' (default: one kernel per axis sweep) or 'staged' (one launch per stage)
     --arith <mode>          'fp64' (default) or 'fp32mixed' (fused engine only)
@@ -41,6 +44,7 @@ Options:
 static bool parse_arguments(ArmonParams& p, int argc, char** argv)
 {
     errno = 0;
+    bool engine_given = false;
     for (int i = 1; i < argc; i++) {
         const char* a = argv[i];
         const char* v = i + 1 < argc ? argv[i + 1] : nullptr;
@@ -152,8 +156,17 @@ static bool parse_arguments(ArmonParams& p, int argc, char** argv)
             else if (strcmp(v, "Y_only") == 0) p.axis_splitting = ARMON_SPLIT_Y_ONLY;
             else { printf("Wrong axis splitting method: %s\n", v); return false; }
             i++;
+        } else if (strcmp(a, "--compare") == 0) {
+            if (!need()) return false;
+            p.compare = (int) strtol(v, nullptr, 2);
+            i++;
+        } else if (strcmp(a, "--comparison-tolerance") == 0) {
+            if (!need()) return false;
+            p.comparison_tolerance = strtod(v, nullptr);
+            i++;
         } else if (strcmp(a, "--engine") == 0) {
             if (!need()) return false;
+            engine_given = true;
             if (strcmp(v, "fused") == 0) p.engine = ARMON_ENGINE_FUSED;
             else if (strcmp(v, "staged") == 0) p.engine = ARMON_ENGINE_STAGED;
             else { printf("Wrong engine: %s\n", v); return false; }
@@ -177,6 +190,8 @@ static bool parse_arguments(ArmonParams& p, int argc, char** argv)
         fprintf(stderr, "Parsing error occurred: %s\n", strerror(errno));
         return false;
     }
+    // the per-step comparison checks every stage: it runs the staged engine unless told otherwise
+    if (p.compare && !engine_given) p.engine = ARMON_ENGINE_STAGED;
     return true;
 }
 

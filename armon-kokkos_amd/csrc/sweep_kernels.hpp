@@ -37,6 +37,21 @@ namespace sweep {
 constexpr int kTile = 256;    // X sweep: threads (cells) per tile
 constexpr int kYBlock = 256;  // Y sweep: columns per block
 
+// Y march: rows per unrolled loop body, and the minimum waves per SIMD the register
+// allocation must allow (0: the compiler's choice).
+#ifndef ARMON_Y_UNROLL
+#define ARMON_Y_UNROLL 4
+#endif
+#ifndef ARMON_Y_WAVES_PER_EU
+#define ARMON_Y_WAVES_PER_EU 0
+#endif
+constexpr int kYUnroll = ARMON_Y_UNROLL;
+#if ARMON_Y_WAVES_PER_EU > 0
+#define ARMON_Y_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ARMON_Y_WAVES_PER_EU)))
+#else
+#define ARMON_Y_OCCUPANCY
+#endif
+
 template <class T>
 struct Cell {
     T rho, un, ut, E, p, c;   // un: velocity along the sweep axis, ut: across it
@@ -139,12 +154,19 @@ __device__ __forceinline__ void reduce_cfl(DeviceState* st, double local)
 // X sweep: LDS-staged row tiles
 // ---------------------------------------------------------------------------------------
 
-// LDS of one X tile: the cell state, the Godunov and final face fluxes, the advection fluxes.
+// LDS of one X tile: the cell state, the final face fluxes, and a 4-array scratch that holds
+// p, c and the Godunov fluxes until the face fluxes are done (stages 0-2), then the advection
+// fluxes (stages 4-5); a barrier separates the last read of the former from the first write
+// of the latter. 10 arrays: 20 KiB in FP64, so that 8 blocks fit in a CU's LDS.
 template <class T>
 struct SmemX {
-    T rho[kTile], un[kTile], ut[kTile], E[kTile], p[kTile], c[kTile];
-    T gu[kTile], gp[kTile], us[kTile], ps[kTile];
-    T f[4][kTile];
+    T rho[kTile], un[kTile], ut[kTile], E[kTile], us[kTile], ps[kTile];
+    T x[4][kTile];
+    __device__ T* p() { return x[0]; }
+    __device__ T* c() { return x[1]; }
+    __device__ T* gu() { return x[2]; }
+    __device__ T* gp() { return x[3]; }
+    __device__ T (&f())[4][kTile] { return x; }
 };
 
 // One X tile: row j, output cells [x0, x0 + W) of the domain described by `a`, with the
@@ -168,20 +190,20 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j,
     const Cell<T> cell = in_range ? load_cell<T, 0, EOS>(a, j, k, a.nx) : benign_cell<T>();
     sm.rho[t] = cell.rho;
     sm.un[t] = cell.un;
-    sm.p[t] = cell.p;
-    sm.c[t] = cell.c;
+    sm.p()[t] = cell.p;
+    sm.c()[t] = cell.c;
     __syncthreads();
 
     // Stage 1-2: face flux at face k (between cells k-1 and k).
     T gu, gp, us, ps;
-    godunov<T>(cell.rho, sm.rho[tm], cell.c, sm.c[tm], cell.un, sm.un[tm], cell.p, sm.p[tm], gu, gp);
+    godunov<T>(cell.rho, sm.rho[tm], cell.c, sm.c()[tm], cell.un, sm.un[tm], cell.p, sm.p()[tm], gu, gp);
     if constexpr (GAD) {
-        sm.gu[t] = gu;
-        sm.gp[t] = gp;
+        sm.gu()[t] = gu;
+        sm.gp()[t] = gp;
         __syncthreads();
         Cell<T> m;
-        m.rho = sm.rho[tm]; m.un = sm.un[tm]; m.p = sm.p[tm]; m.c = sm.c[tm];
-        face_flux_gad<T, FLUX - 1>(dt, dx, m, cell, sm.gu[tm], sm.gp[tm], gu, gp, sm.gu[tp], sm.gp[tp], us, ps);
+        m.rho = sm.rho[tm]; m.un = sm.un[tm]; m.p = sm.p()[tm]; m.c = sm.c()[tm];
+        face_flux_gad<T, FLUX - 1>(dt, dx, m, cell, sm.gu()[tm], sm.gp()[tm], gu, gp, sm.gu()[tp], sm.gp()[tp], us, ps);
     } else {
         us = gu;
         ps = gp;
@@ -223,10 +245,10 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j,
         const int td = disp > 0 ? tm : t;
         advection_flux_1<T>(disp, sm.rho[td], sm.un[td], sm.ut[td], sm.E[td], f0, f1, f2, f3);
     }
-    sm.f[0][t] = f0;
-    sm.f[1][t] = f1;
-    sm.f[2][t] = f2;
-    sm.f[3][t] = f3;
+    sm.f()[0][t] = f0;
+    sm.f()[1][t] = f1;
+    sm.f()[2][t] = f2;
+    sm.f()[3][t] = f3;
     __syncthreads();
 
     // Stage 5: projection of cell k, output.
@@ -234,11 +256,11 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j,
     if (t >= R && t < R + W && k < a.nx) {
         const T inv_dx = (T) a.inv_dx;
         if (a.dx_pow2)
-            projection<true, T>(dx, inv_dx, dt, us, us_p, f0, sm.f[0][tp], f1, sm.f[1][tp], f2, sm.f[2][tp], f3,
-                                sm.f[3][tp], rho, un, ut, E);
+            projection<true, T>(dx, inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
+                                sm.f()[3][tp], rho, un, ut, E);
         else
-            projection<false, T>(dx, inv_dx, dt, us, us_p, f0, sm.f[0][tp], f1, sm.f[1][tp], f2, sm.f[2][tp], f3,
-                                 sm.f[3][tp], rho, un, ut, E);
+            projection<false, T>(dx, inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
+                                 sm.f()[3][tp], rho, un, ut, E);
         const int idx = a.index_start + j * a.row_length + k;
         a.out_rho[idx] = rho;
         a.out_u[idx] = un;
@@ -303,7 +325,7 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
 
     double local = DBL_MAX;
 
-#pragma unroll 4
+#pragma unroll kYUnroll
     for (int n = y0 - R; n < y1 + R; ++n) {
         const Cell<T> cn = lane_ok ? load_cell<T, 1, EOS>(a, x, n, a.ny) : benign_cell<T>();
         pre[0] = pre[1];
@@ -394,7 +416,7 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
 }
 
 template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kYBlock) k_sweep_y(SweepArgs<T> a, int ncolblk)
+__global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs<T> a, int ncolblk)
 {
     if (!a.st->active) return;
     const T dt = (T) (a.st->current_dt * a.dt_factor);
@@ -461,11 +483,14 @@ __global__ void __launch_bounds__(kTile) k_batch_x(BatchArgs<T> b, const BatchIt
 }
 
 template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kYBlock) k_batch_y(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
+__global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY
+        k_batch_y(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
 {
     const int lane = threadIdx.x & 63;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int it = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; it < nitems; it += nwaves) {
+    // the item index is uniform across the wave: keep it (and what is loaded through it) scalar
+    const int first = __builtin_amdgcn_readfirstlane((int) ((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    for (int it = first; it < nitems; it += nwaves) {
         const BatchItemY item = items[it];
         const DeviceState* st = b.states + item.domain;
         if (!st->active) continue;   // the same for the whole wave

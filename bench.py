@@ -61,6 +61,39 @@ def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles, graph_c
     return elapsed, kt, rho
 
 
+def measure_batch(armon, n_domains, seed, steps, warmup, precision):
+    """The seeded ensemble of uneven domains, advanced together (one launch per sweep)."""
+    specs = armon.random_specs(seed, n_domains)
+    for sp in specs:
+        sp.max_time = 1e9   # every domain stays active through the timed cycles
+    common = armon.batch_params(max_cycles=10**9, graph_chunk=16, precision=precision, verbose=5, **WORKLOAD)
+    b = armon.Batch(common, specs)
+    b.init()
+    tuning = b.autotune()
+    b.begin()
+    lib = armon.load()
+    stream = b.stream()
+
+    def sync():
+        lib.armon_set_stream(stream)
+        lib.armon_synchronize()
+        lib.armon_set_stream(None)
+
+    b.advance(warmup)
+    sync()
+    t0 = time.perf_counter()
+    b.advance(steps)
+    sync()
+    elapsed = time.perf_counter() - t0
+    kt = b.kernel_times(4)
+    if b.finish() != 0:
+        raise RuntimeError(f"batch run failed: {armon.errors()}")
+    cells = b.cells()
+    tests = [int(sp.test) for sp in specs]
+    b.close()
+    return elapsed, cells, kt, tuning, tests
+
+
 def cpu_baseline(nx, ny, cycles):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -85,6 +118,9 @@ def main():
     ap.add_argument("--graph-chunk", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-domains", type=int, default=256)
+    ap.add_argument("--batch-seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -131,6 +167,32 @@ def main():
         log(f"[bench] FP32-mixed: {rate32 / 1e9:.3f} G cell-updates/s, speedup {rate32 / rate64:.2f}, "
             f"rho L1 {l1:.2e}")
 
+    batched = None
+    if not args.no_batch:
+        batched = {}
+        for prec, key in ((0, "fp64"), (1, "fp32_mixed")):
+            log(f"[bench] batched ensemble {key}: {args.batch_domains} domains (seed {args.batch_seed})")
+            el, bcells, bkt, tun, tests = measure_batch(armon, args.batch_domains, args.batch_seed, args.steps,
+                                                        args.warmup, prec)
+            rate = bcells * args.steps / el
+            avg = {names[k]: bkt.total_ms[k] / max(bkt.launches[k], 1) for k in range(3)}
+            word = 8 if prec == 0 else 4
+            batched[key] = {
+                "value": rate, "unit": "cell-updates/s", "ms_per_step": el / args.steps * 1e3,
+                "sweep_ms": avg,
+                "sweep_GBps": {k: 8 * word * bcells / (avg[k] * 1e-3) / 1e9 for k in ("sweep_x", "sweep_y")},
+                "tuning": {"y_run_rows": tun.y_run_rows, "grid_x": tun.grid_x, "grid_y": tun.grid_y,
+                           "items_x": tun.items_x, "items_y": tun.items_y},
+            }
+            log(f"[bench] batched {key}: {rate / 1e9:.3f} G cell-updates/s over {bcells} cells, "
+                f"{el / args.steps * 1e3:.3f} ms/cycle, tuning L={tun.y_run_rows} gx={tun.grid_x} gy={tun.grid_y}")
+        batched["domains"] = args.batch_domains
+        batched["cells"] = bcells
+        batched["tests"] = {"Sod": tests.count(0), "Sod_circ": tests.count(2), "Sedov": tests.count(4)}
+        batched["workload"] = (f"{args.batch_domains} domains from splitmix64 seed {args.batch_seed}: area "
+                               f"log-uniform 64^2..1024^2, aspect 1..4, Sod/Sod_circ/Sedov, states x U(0.8, 1.2), "
+                               f"same scheme and CFL as the single domain")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle fast-math build, Sedov {n}x{n}, {args.cpu_cycles} cycles")
@@ -161,6 +223,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
                      "kernel_avg_ms": sweep_avg, "copy_GBps": copy_gbps},
         "fp32_mixed": fp32,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

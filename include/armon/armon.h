@@ -123,6 +123,9 @@ typedef struct ArmonParams {
     int engine;                     /* ARMON_ENGINE_* */
     int precision;                  /* ARMON_PREC_* (fused engine only) */
     int graph_chunk;                /* cycles per captured graph launch (fused engine); 0: no graph */
+    /* per-step comparison with reference files (parameters.h:95-97; staged engine only) */
+    int compare;
+    double comparison_tolerance;    /* relative, default 1e-13 */
 } ArmonParams;
 
 ARMON_EXPORT void armon_params_default(ArmonParams* p);              /* parameters.h:34-97 defaults */
@@ -167,7 +170,8 @@ enum { ARMON_FIELD_X = 0, ARMON_FIELD_Y, ARMON_FIELD_RHO, ARMON_FIELD_UMAT, ARMO
        ARMON_FIELD_DOMAIN_MASK, ARMON_FIELD_COUNT };
 
 typedef struct ArmonResult {
-    int status;          /* 0 ok, 1 invalid time step (the reference exits), 2 other error */
+    int status;          /* 0 ok, 1 invalid time step (the reference exits), 2 other error,
+                            3 the per-step comparison found a difference (the loop stopped there) */
     int cycles;
     double time;
     double current_dt;   /* dt of the last cycle */
