@@ -30,6 +30,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a FP64 sweep at 8192^2, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "sweeps_sedov8192_hbm_traffic_valu.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                vals = dict(kv.split("=") for kv in line.split()[1:])
+                return float(vals[kernel])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles, graph_chunk):
     p = armon.make_params("Sedov", nx=nx, ny=ny, max_cycles=10**9, max_time=1e9, verbose=5, engine=1,
                           precision=precision, graph_chunk=graph_chunk, **WORKLOAD)
@@ -219,7 +233,8 @@ def main():
                                f"splitting, CFL 0.7, fused engine, hipGraph of {args.graph_chunk} cycles",
                    "cells": cells, "engine": "fused"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": names[dom],
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(names[dom]) if n == 8192 else None, "kernel": names[dom],
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_s * 1e3,
                      "kernel_avg_ms": sweep_avg, "copy_GBps": copy_gbps},
         "fp32_mixed": fp32,
