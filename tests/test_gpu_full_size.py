@@ -1,0 +1,91 @@
+"""Parity at the benchmark's sizes and over whole runs.
+
+- Sedov 8192x8192 (the benchmark configuration) for 20 cycles: every field bit-identical to
+  the oracle.
+- Sedov 256x256 run to its end time t = 1 (thousands of cycles): identical cycle count, final
+  dt and state.
+- Bizarrium 1024x1024 for 200 cycles: identical.
+- Size-independent properties at 8192x8192: mass and energy conserved to 1e-12 over 100
+  cycles (FP64 fused), and the FP32-mixed form of circular Sod after 200 cycles within the
+  density L1 bound of 1e-5 against the FP64 engine."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("rho", "umat", "vmat", "Emat", "pmat", "cmat")
+BENCH = dict(scheme=1, projection=2, limiter=1, axis_splitting=0, nb_ghosts=4, cfl=0.7, verbose=5)
+
+
+def compare_run(armon, po, oracle, test, **cfg):
+    print(f"case {test} {cfg}: oracle", flush=True)
+    p = po.make_params(oracle, test, **cfg)
+    st, od, cycles, final_dt, _ = po.run(oracle, p)
+    assert st == 0
+    print(f"case {test}: GPU", flush=True)
+    q = armon.make_params(test, engine=1, **cfg)
+    r, s = armon.run(q)
+    assert r.status == 0, armon.errors()
+    assert (r.cycles, r.final_dt) == (cycles, final_dt)
+    mask = od.array("domain_mask") == 1
+    bad = []
+    for f in STATE:
+        a, b = s.read(f)[mask], od.array(f)[mask]
+        if not np.array_equal(a.view(np.uint64), b.view(np.uint64)):
+            bad.append(f"{f}: {int(np.count_nonzero(a != b))} cells differ, max {np.max(np.abs(a - b)):.3e}")
+    s.close()
+    del od
+    assert not bad, "; ".join(bad)
+    return cycles
+
+
+def test_sedov_8192_20_cycles(armon, po, oracle):
+    assert compare_run(armon, po, oracle, "Sedov", nx=8192, ny=8192, max_cycles=20, **BENCH) == 20
+
+
+def test_sedov_256_to_end_time(armon, po, oracle):
+    cycles = compare_run(armon, po, oracle, "Sedov", nx=256, ny=256, max_cycles=10**6, **BENCH)
+    print(f"Sedov 256^2 reached t = 1 in {cycles} cycles", flush=True)
+    assert cycles > 1000
+
+
+def test_bizarrium_1024_200_cycles(armon, po, oracle):
+    cfg = dict(BENCH, cfl=0.6, max_time=1.0)
+    assert compare_run(armon, po, oracle, "Bizarrium", nx=1024, ny=1024, max_cycles=200, **cfg) == 200
+
+
+def test_conservation_8192(armon):
+    p = armon.make_params("Sedov", nx=8192, ny=8192, max_cycles=100, engine=1, **BENCH)
+    s = armon.Solver(p)
+    s.init_test()
+    lib = armon.load()
+    s.begin()
+    m0, e0 = C.c_double(), C.c_double()
+    assert lib.armon_solver_conservation(s.h, C.byref(m0), C.byref(e0)) == 0
+    s.advance(100)
+    r = s.finish()
+    m1, e1 = C.c_double(), C.c_double()
+    assert lib.armon_solver_conservation(s.h, C.byref(m1), C.byref(e1)) == 0
+    assert r.status == 0 and r.cycles == 100
+    assert abs(m1.value - m0.value) <= 1e-12 * abs(m0.value)
+    assert abs(e1.value - e0.value) <= 1e-12 * abs(e0.value)
+    s.close()
+
+
+def test_fp32_mixed_sod_circ_8192(armon):
+    cfg = dict(BENCH, nx=8192, ny=8192, max_cycles=200, max_time=1e9)
+    print("case Sod_circ 8192^2 200 cycles FP64 / FP32-mixed", flush=True)
+    r64, s64 = armon.run(armon.make_params("Sod_circ", engine=1, **cfg))
+    r32, s32 = armon.run(armon.make_params("Sod_circ", engine=1, precision=1, **cfg))
+    assert r64.status == 0 and r32.status == 0 and r64.cycles == r32.cycles == 200
+    mask = s64.read("domain_mask") == 1
+    errs = {}
+    for f in ("rho", "umat", "vmat", "Emat", "pmat"):
+        a, b = s32.read(f)[mask], s64.read(f)[mask]
+        errs[f] = (float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))), float(np.max(np.abs(a - b))))
+    print("FP32-mixed vs FP64 (rel L1, Linf):", errs, flush=True)
+    assert errs["rho"][0] <= 1e-5
+    s64.close()
+    s32.close()
