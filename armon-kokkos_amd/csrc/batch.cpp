@@ -120,19 +120,25 @@ void fail(const char* where, const std::exception& e)
     else report_std(where, e);
 }
 
-// Work lists. X: every (row, tile) of every domain. Y: every (64-column group, run of at most
-// `run` rows) of every domain, longest runs first so that a grid-stride walk over them
-// balances well.
+// Work lists. X: every tile of every domain's stream of padded rows (the tile mapping of
+// sweep_x_tile: tiles of W outputs overlapping by 2R slots). Y: every (64-column group, run of
+// at most `run` rows) of every domain, longest runs first so that a grid-stride walk over
+// them balances well.
 void build_items(ArmonBatch* b, int run)
 {
     const int W = x_tile_outputs(b->var);
+    const int R = read_extent(b->var.flux, b->var.proj);
     b->items_x.clear();
     b->items_y.clear();
     const int n = (int) b->p.size();
     for (int d = 0; d < n; d++) {
         const ArmonParams& q = b->p[d];
-        for (int j = 0; j < q.ny; j++)
-            for (int x0 = 0; x0 < q.nx; x0 += W) b->items_x.push_back(BatchItemX{d, j, x0, 0});
+        const int64_t slots = q.nx + 2 * R;
+        const int64_t tiles = (slots * q.ny - 2 * R + W - 1) / W;
+        for (int64_t t = 0; t < tiles; t++) {
+            const int64_t s0 = t * W;
+            b->items_x.push_back(BatchItemX{d, (int) (s0 / slots), (int) (s0 % slots), 0});
+        }
         for (int x0 = 0; x0 < q.nx; x0 += kWave)
             for (int y0 = 0; y0 < q.ny; y0 += run) b->items_y.push_back(BatchItemY{d, x0, y0, std::min(y0 + run, q.ny)});
     }
@@ -480,6 +486,8 @@ ARMON_EXPORT ArmonBatch* armon_batch_create(const ArmonParams* common, int n, co
             D.cfl_dx = D.dx[0];
             D.cfl_dy = D.dx[1];
             for (int k = 0; k < 4; k++) armon_test_boundary_condition(q.test, sides[k], &D.bc[k][0], &D.bc[k][1]);
+            D.x_slots = q.nx + 2 * read_extent(b->var.flux, b->var.proj);
+            D.x_inv_slots = 1.0f / (float) D.x_slots;
             cfg[d] = TimeStepConfig{q.cfl, q.Dt, q.max_time, q.max_cycles, q.cst_dt, q.dt_on_even_cycles};
         }
         b->d_domains = device_alloc<BatchDomain>(n, "hipMalloc(domains)");

@@ -358,6 +358,7 @@ bool fused_supported(const ArmonParams& p, std::string& why)
         return false;
     }
     if ((int64_t) p.nb_cells >= (int64_t) 1 << 31) { why = "more than 2^31 cells"; return false; }
+    if (p.nx >= 1 << 23) { why = "rows of 2^23 cells or more"; return false; }   // X tiles' slot arithmetic
     return true;
 }
 

@@ -35,6 +35,8 @@ struct SweepArgs {
     DeviceState* st;
     int nx, ny, row_length, index_start, stencil_width;
     int seg_len;                          // Y sweep: rows per marching run (set by the launcher)
+    int x_slots;                          // X sweep: slots per row, nx + 2 x the ghost layers read
+    float x_inv_slots;                    // 1 / x_slots
     int dx_pow2, reduce;
     double gamma, dx, inv_dx, dt_factor;
     double lo_u, lo_v, hi_u, hi_v;        // BC factors: Left/Bottom (lo), Right/Top (hi)
@@ -69,11 +71,13 @@ struct BatchDomain {
     double dx[2], inv_dx[2];       // per axis
     double cfl_dx, cfl_dy;
     double bc[4][2];               // per side (Left, Right, Top, Bottom): u, v factors
+    int x_slots;                   // X sweep: slots per row (nx + 2 x the ghost layers read)
+    float x_inv_slots;             // 1 / x_slots
 };
 
-// Work items: an X tile (domain, row, first output column) or a Y run of one wave
-// (domain, first column, output rows [y0, y1)).
-struct BatchItemX { int domain, row, x0, pad; };
+// Work items: an X tile (domain; row and slot within that row of the tile's first thread,
+// see sweep_x_tile) or a Y run of one wave (domain, first column, output rows [y0, y1)).
+struct BatchItemX { int domain, row, slot, pad; };
 struct BatchItemY { int domain, x0, y0, y1; };
 
 template <class T>

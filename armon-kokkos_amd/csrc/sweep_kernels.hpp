@@ -169,10 +169,17 @@ struct SmemX {
     __device__ T (&f())[4][kTile] { return x; }
 };
 
-// One X tile: row j, output cells [x0, x0 + W) of the domain described by `a`, with the
-// cycle's dt. Called by all threads of a kTile-thread block.
+// One X tile of the domain described by `a`, with the cycle's dt. Called by all threads of a
+// kTile-thread block.
+//
+// The rows of the domain are laid end to end as a stream of slots, each row taking
+// x_slots = nx + 2R slots (its R ghost cells on each side, then the next row). A tile covers
+// kTile consecutive slots, starting at slot `k0` of row `j0`; consecutive tiles overlap by 2R
+// slots, and the slots [R, kTile - R) of a tile that hold real cells are its outputs. A cell's
+// result depends only on the cells within R of it in its own row, so a tile may span the end
+// of one row and the start of the next; no thread is spent on a tile's share of a row end.
 template <class T, int FLUX, int PROJ, int EOS>
-__device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j, int x0, SmemX<T>& sm)
+__device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0, int k0, SmemX<T>& sm)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
     constexpr int E2 = PROJ == 2 ? 1 : 0;
@@ -181,8 +188,23 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j,
     const T dx = (T) a.dx;
 
     const int t = threadIdx.x;
-    const int k = x0 - R + t;
-    const bool in_range = k < a.nx + R;   // k >= -R always holds
+    // row and cell of this thread's slot: n = k0 + t < x_slots + kTile. Rows of at least kTile
+    // slots: n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and
+    // a single correction makes it exact.
+    const int n = k0 + t;
+    int dj, r;
+    if (a.x_slots >= kTile) {
+        dj = n >= a.x_slots ? 1 : 0;
+        r = n - (dj ? a.x_slots : 0);
+    } else {
+        dj = (int) ((float) n * a.x_inv_slots);
+        r = n - dj * a.x_slots;
+        if (r < 0) { dj--; r += a.x_slots; }
+        else if (r >= a.x_slots) { dj++; r -= a.x_slots; }
+    }
+    const int j = j0 + dj;
+    const int k = r - R;                  // -R <= k < nx + R
+    const bool in_range = j < a.ny;       // slots past the last row
     const int tm = t > 0 ? t - 1 : 0;
     const int tp = t < kTile - 1 ? t + 1 : kTile - 1;
 
@@ -253,7 +275,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j,
 
     // Stage 5: projection of cell k, output.
     double local = DBL_MAX;
-    if (t >= R && t < R + W && k < a.nx) {
+    if (t >= R && t < R + W && in_range && k >= 0 && k < a.nx) {
         const T inv_dx = (T) a.inv_dx;
         if (a.dx_pow2)
             projection<true, T>(dx, inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
@@ -272,14 +294,15 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j,
 }
 
 template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a, int nseg)
+__global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a)
 {
     constexpr int W = kTile - 2 * (2 + (FLUX > 0 ? 1 : 0) + (PROJ == 2 ? 1 : 0));
     if (!a.st->active) return;
     const T dt = (T) (a.st->current_dt * a.dt_factor);
     __shared__ SmemX<T> sm;
-    const int j = blockIdx.x / nseg;
-    sweep_x_tile<T, FLUX, PROJ, EOS>(a, dt, j, (blockIdx.x - j * nseg) * W, sm);
+    const int s0 = blockIdx.x * W;        // first slot of the tile
+    const int j0 = s0 / a.x_slots;
+    sweep_x_tile<T, FLUX, PROJ, EOS>(a, dt, j0, s0 - j0 * a.x_slots, sm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -465,6 +488,8 @@ __device__ __forceinline__ SweepArgs<T> batch_domain_args(const BatchArgs<T>& b,
     a.hi_v = D.bc[hi][1];
     a.cfl_dx = D.cfl_dx;
     a.cfl_dy = D.cfl_dy;
+    a.x_slots = D.x_slots;
+    a.x_inv_slots = D.x_inv_slots;
     return a;
 }
 
@@ -477,7 +502,7 @@ __global__ void __launch_bounds__(kTile) k_batch_x(BatchArgs<T> b, const BatchIt
         const DeviceState* st = b.states + item.domain;
         if (!st->active) continue;   // the same for the whole block
         const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
-        sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.x0, sm);
+        sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.slot, sm);
         __syncthreads();             // the next item reuses the LDS
     }
 }
@@ -499,6 +524,10 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY
         sweep_y_run<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0, item.y1);
     }
 }
+
+// X tiles covering a stream of `slots` slots (tiles of W outputs overlapping by 2R): the last
+// real cell sits R slots before the end.
+inline int64_t x_tiles(int64_t slots, int R, int W) { return (slots - 2 * R + W - 1) / W; }
 
 inline unsigned grid_for(int64_t n, int block, int64_t max_blocks)
 {
@@ -543,8 +572,11 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
     constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
     if (axis == 0) {
         constexpr int W = kTile - 2 * R;
-        const int nseg = (a.nx + W - 1) / W;
-        k_sweep_x<T, F, P, E><<<dim3((unsigned) (nseg * a.ny)), dim3(kTile), 0, s>>>(a, nseg);
+        SweepArgs<T> b = a;
+        b.x_slots = a.nx + 2 * R;
+        b.x_inv_slots = 1.0f / (float) b.x_slots;
+        const int64_t tiles = x_tiles((int64_t) a.ny * b.x_slots, R, W);
+        k_sweep_x<T, F, P, E><<<dim3((unsigned) tiles), dim3(kTile), 0, s>>>(b);
         check_launch("sweep_x");
     } else {
         // One resident round of blocks: each block marches down a contiguous run of rows of
