@@ -507,6 +507,19 @@ __global__ void __launch_bounds__(kTile) k_batch_x(BatchArgs<T> b, const BatchIt
     }
 }
 
+// One block per X item (the launch configuration the autotuner usually picks): no item loop,
+// so the tile body keeps the single-domain kernel's register budget.
+template <class T, int FLUX, int PROJ, int EOS>
+__global__ void __launch_bounds__(kTile) k_batch_x1(BatchArgs<T> b, const BatchItemX* __restrict__ items)
+{
+    __shared__ SmemX<T> sm;
+    const BatchItemX item = items[blockIdx.x];
+    const DeviceState* st = b.states + item.domain;
+    if (!st->active) return;
+    const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
+    sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.slot, sm);
+}
+
 template <class T, int FLUX, int PROJ, int EOS>
 __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY
         k_batch_y(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
@@ -611,7 +624,9 @@ template <class T, int F, int P, int E>
 void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, int nitems, int grid, hipStream_t s)
 {
     if (nitems <= 0) return;
-    if (axis == 0)
+    if (axis == 0 && grid >= nitems)
+        k_batch_x1<T, F, P, E><<<dim3((unsigned) nitems), dim3(kTile), 0, s>>>(b, static_cast<const BatchItemX*>(items));
+    else if (axis == 0)
         k_batch_x<T, F, P, E><<<dim3((unsigned) grid), dim3(kTile), 0, s>>>(b, static_cast<const BatchItemX*>(items),
                                                                              nitems);
     else
