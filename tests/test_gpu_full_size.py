@@ -4,7 +4,9 @@
   the oracle.
 - Sedov 256x256 run to its end time t = 1 (thousands of cycles): identical cycle count, final
   dt and state.
-- Bizarrium 1024x1024 for 200 cycles: identical.
+- Bizarrium 1024x1024 for 200 cycles (minmod) and 500 steps (superbee): identical.
+- The 1D tubes at full size (Bizarrium 4096x4096 for 500 steps, Sod 8192x8192 for 200):
+  every row stays equal to the first.
 - Size-independent properties at 8192x8192: mass and energy conserved to 1e-12 over 100
   cycles (FP64 fused), and the FP32-mixed form of circular Sod after 200 cycles within the
   density L1 bound of 1e-5 against the FP64 engine."""
@@ -54,6 +56,34 @@ def test_sedov_256_to_end_time(armon, po, oracle):
 def test_bizarrium_1024_200_cycles(armon, po, oracle):
     cfg = dict(BENCH, cfl=0.6, max_time=1.0)
     assert compare_run(armon, po, oracle, "Bizarrium", nx=1024, ny=1024, max_cycles=200, **cfg) == 200
+
+
+def test_bizarrium_superbee_1024_500_steps(armon, po, oracle):
+    """The Bizarrium shock tube of the benchmark's parity set (GAD + superbee, 500 steps), at
+    1024^2 so that the oracle finishes in seconds."""
+    cfg = dict(BENCH, limiter=2, cfl=0.6, max_time=1.0)
+    assert compare_run(armon, po, oracle, "Bizarrium", nx=1024, ny=1024, max_cycles=500, **cfg) == 500
+
+
+@pytest.mark.parametrize("test,n", [("Bizarrium", 4096), ("Sod", 8192)])
+def test_one_dimensional_tubes_stay_row_invariant(armon, test, n):
+    """At full size: the Bizarrium and Sod tubes vary only along x and the y walls reflect, so
+    every row must stay equal to the first one (the zero y velocity may change sign)."""
+    steps = 500 if test == "Bizarrium" else 200
+    cfg = dict(BENCH, limiter=2 if test == "Bizarrium" else 1, cfl=0.6 if test == "Bizarrium" else 0.95,
+               max_time=1.0, nx=n, ny=n, max_cycles=steps)
+    print(f"case {test} {n}^2 {steps} steps: row invariance", flush=True)
+    r, s = armon.run(armon.make_params(test, engine=1, **cfg))
+    assert r.status == 0 and r.cycles == steps, armon.errors()
+    p = s.params()
+    g, rl = p.nb_ghosts, p.row_length
+    for f in ("rho", "umat", "vmat", "Emat", "pmat"):
+        a = s.read(f).reshape(-1, rl)[g:g + n, g:g + n]
+        assert np.all(np.isfinite(a)), f
+        assert np.array_equal(a, np.broadcast_to(a[0], a.shape)), f   # == treats -0 and +0 alike
+    rho = s.read("rho").reshape(-1, rl)[g, g:g + n]
+    assert rho.max() > rho.min()   # the tube has evolved, not just stayed put
+    s.close()
 
 
 def test_conservation_8192(armon):
