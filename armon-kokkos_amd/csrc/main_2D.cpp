@@ -2,6 +2,8 @@
 // options of this implementation (--engine, --arith, --graph-chunk).
 #include "armon/armon.h"
 
+#include <chrono>
+
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -39,9 +41,19 @@ Options:
     --engine <engine>       'fused' (default: one kernel per axis sweep) or 'staged' (one launch per stage)
     --arith <mode>          'fp64' (default) or 'fp32mixed' (fused engine only)
     --graph-chunk N         Cycles per captured hipGraph launch (fused engine, even; 0 disables)
+    --batch N               Run a batched ensemble of N domains (64^2 to 1024^2 cells, Sod / Sod_circ /
+                            Sedov, perturbed states) drawn from --seed S (default 2024), each to its own
+                            end time or --cycle; --autotune 0-1 (default 1)
 )";
 
-static bool parse_arguments(ArmonParams& p, int argc, char** argv)
+// Batched-ensemble mode (--batch N): N domains drawn from --seed, advanced together.
+struct BatchOptions {
+    int domains = 0;
+    unsigned long long seed = 2024;
+    int autotune = 1;
+};
+
+static bool parse_arguments(ArmonParams& p, BatchOptions& bo, int argc, char** argv)
 {
     errno = 0;
     bool engine_given = false;
@@ -181,6 +193,18 @@ static bool parse_arguments(ArmonParams& p, int argc, char** argv)
             if (!need()) return false;
             p.graph_chunk = (int) strtol(v, nullptr, 10);
             i++;
+        } else if (strcmp(a, "--batch") == 0) {
+            if (!need()) return false;
+            bo.domains = (int) strtol(v, nullptr, 10);
+            i++;
+        } else if (strcmp(a, "--seed") == 0) {
+            if (!need()) return false;
+            bo.seed = strtoull(v, nullptr, 10);
+            i++;
+        } else if (strcmp(a, "--autotune") == 0) {
+            if (!need()) return false;
+            bo.autotune = (int) strtol(v, nullptr, 2);
+            i++;
         } else {
             fprintf(stderr, "Wrong option: %s\n", a);
             return false;
@@ -195,11 +219,65 @@ static bool parse_arguments(ArmonParams& p, int argc, char** argv)
     return true;
 }
 
+// The ensemble: domains of 64^2 to 1024^2 cells from the seed, each run to its own end time
+// (or --cycle cycles) with the scheme options of the command line.
+static int run_batch(ArmonParams common, const BatchOptions& bo)
+{
+    const int n = bo.domains;
+    ArmonDomainSpec* specs = new ArmonDomainSpec[n];
+    int status = 1;
+    ArmonBatch* b = nullptr;
+    if (armon_batch_random_specs(bo.seed, n, 64 * 64, 1024 * 1024, 4.0, specs) != 0) goto done;
+    common.engine = ARMON_ENGINE_FUSED;
+    b = armon_batch_create(&common, n, specs);
+    if (!b) goto done;
+    if (armon_batch_init(b) != 0) goto done;
+    if (bo.autotune) {
+        ArmonBatchTuning t;
+        if (armon_batch_autotune(b, &t) != 0) goto done;
+        if (common.verbose < 3)
+            printf("Autotuned: Y runs of %d rows, X grid %d, Y grid %d (%.3f + %.3f ms per sweep pair)\n",
+                   t.y_run_rows, t.grid_x, t.grid_y, t.ms_x, t.ms_y);
+    }
+    {
+        if (armon_batch_begin(b) != 0) goto done;
+        const auto t0 = std::chrono::steady_clock::now();
+        const int run_status = armon_batch_run(b);   // returns once every domain has stopped
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        const int finish_status = armon_batch_finish(b);
+        if (run_status != 0 || finish_status != 0) goto done;
+        double updates = 0;
+        int max_cycles = 0;
+        for (int d = 0; d < n; d++) {
+            ArmonResult r;
+            ArmonParams q;
+            armon_batch_domain_result(b, d, &r);
+            armon_batch_domain_params(b, d, &q);
+            updates += (double) r.cycles * q.nx * q.ny;
+            if (r.cycles > max_cycles) max_cycles = r.cycles;
+            if (common.verbose < 2)
+                printf("domain %3d: test %d, %4dx%-4d, %6d cycles, t = %.6g, final dt = %.6g\n", d, q.test, q.nx,
+                       q.ny, r.cycles, r.time, r.final_dt);
+        }
+        if (common.verbose < 4)
+            printf("\nBatch of %d domains (%lld cells): %.4g s, up to %d cycles, %.4g Mega cell-updates/sec\n", n,
+                   (long long) armon_batch_cells(b), ms * 1e-3, max_cycles, updates / (ms * 1e-3) / 1e6);
+        status = 0;
+    }
+done:
+    if (status != 0) fprintf(stderr, "batch run failed: %s\n", armon_last_error());
+    if (b) armon_batch_destroy(b);
+    delete[] specs;
+    return status;
+}
+
 int main(int argc, char* argv[])
 {
     ArmonParams p;
     armon_params_default(&p);
-    if (!parse_arguments(p, argc, argv)) return 1;
+    BatchOptions bo;
+    if (!parse_arguments(p, bo, argc, argv)) return 1;
+    if (bo.domains > 0) return run_batch(p, bo);
     if (armon_params_init(&p) != 0) return 1;
     if (!armon_params_check(&p)) return 1;
     if (p.verbose < 3) {
