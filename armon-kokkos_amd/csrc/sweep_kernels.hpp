@@ -52,6 +52,20 @@ constexpr int kYUnroll = ARMON_Y_UNROLL;
 #define ARMON_Y_OCCUPANCY
 #endif
 
+// MI355X dispatches the blocks of a grid round-robin over its 8 XCDs (block b runs on XCD
+// b % 8), each XCD with its own L2. xcd_block renumbers blocks so that every XCD works through
+// one contiguous eighth of the logical blocks: neighbouring tiles (X) and the runs of
+// neighbouring rows (Y) then share an L2, which serves their overlapping halo cells, and
+// each XCD streams its own contiguous part of the arrays. A trailing partial group of 8
+// keeps its numbers.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int q = nb / kXcds;
+    return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
+}
+
 template <class T>
 struct Cell {
     T rho, un, ut, E, p, c;   // un: velocity along the sweep axis, ut: across it
@@ -300,7 +314,7 @@ __global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a)
     if (!a.st->active) return;
     const T dt = (T) (a.st->current_dt * a.dt_factor);
     __shared__ SmemX<T> sm;
-    const int s0 = blockIdx.x * W;        // first slot of the tile
+    const int s0 = xcd_block(blockIdx.x, gridDim.x) * W;   // first slot of the tile
     const int j0 = s0 / a.x_slots;
     sweep_x_tile<T, FLUX, PROJ, EOS>(a, dt, j0, s0 - j0 * a.x_slots, sm);
 }
@@ -443,8 +457,9 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
 {
     if (!a.st->active) return;
     const T dt = (T) (a.st->current_dt * a.dt_factor);
-    const int seg = blockIdx.x / ncolblk;
-    const int x = (blockIdx.x - seg * ncolblk) * kYBlock + threadIdx.x;
+    const int bid = xcd_block(blockIdx.x, gridDim.x);   // runs in segment-major order
+    const int seg = bid / ncolblk;
+    const int x = (bid - seg * ncolblk) * kYBlock + threadIdx.x;
     const int y0 = seg * a.seg_len;
     sweep_y_run<T, FLUX, PROJ, EOS>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
 }
@@ -513,7 +528,7 @@ template <class T, int FLUX, int PROJ, int EOS>
 __global__ void __launch_bounds__(kTile) k_batch_x1(BatchArgs<T> b, const BatchItemX* __restrict__ items)
 {
     __shared__ SmemX<T> sm;
-    const BatchItemX item = items[blockIdx.x];
+    const BatchItemX item = items[xcd_block(blockIdx.x, gridDim.x)];
     const DeviceState* st = b.states + item.domain;
     if (!st->active) return;
     const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
@@ -560,14 +575,18 @@ inline int resident_blocks(const void* kernel, int block)
     return per_cu * cus;
 }
 
-// Rows per marching run: the column blocks times the runs per column fill one resident round,
-// but a run is kept at least 16 halo widths long so that its warm-up stays small.
-inline int y_run_rows(int ny, int ncolblk, int resident, int halo)
+// Rows per marching run of the Y sweep: about kYRun rows, with the number of runs per column
+// a multiple of the XCD count when the column is long enough, so that each XCD's contiguous
+// share of the blocks (xcd_block) is a set of whole row bands; never below 16 halo widths,
+// so that the 2-halo warm-up of a run stays small.
+constexpr int kYRun = 128;
+
+inline int y_run_rows(int ny, int halo)
 {
-    int nseg = resident > 0 ? resident / (ncolblk > 0 ? ncolblk : 1) : 1;
-    if (nseg < 1) nseg = 1;
-    int len = (ny + nseg - 1) / nseg;
     const int min_len = 16 * halo;
+    int nseg = (ny + kYRun - 1) / kYRun;
+    if (nseg >= kXcds) nseg = (nseg + kXcds - 1) / kXcds * kXcds;
+    int len = (ny + nseg - 1) / nseg;
     if (len < min_len) len = min_len;
     if (len > ny) len = ny;
     return len > 0 ? len : 1;
@@ -592,14 +611,11 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
         k_sweep_x<T, F, P, E><<<dim3((unsigned) tiles), dim3(kTile), 0, s>>>(b);
         check_launch("sweep_x");
     } else {
-        // One resident round of blocks: each block marches down a contiguous run of rows of
-        // its 256 columns, so the warm-up is paid once per run and no partial second round
-        // of blocks leaves CUs idle.
-        static int resident = 0;
-        if (resident == 0) resident = resident_blocks((const void*) k_sweep_y<T, F, P, E>, kYBlock);
+        // Each block marches down a run of rows of its 256 columns (the warm-up is paid once
+        // per run); blocks are numbered row band by row band (segment-major).
         const int ncolblk = (a.nx + kYBlock - 1) / kYBlock;
         SweepArgs<T> b = a;
-        b.seg_len = y_run_rows(a.ny, ncolblk, resident, R);
+        b.seg_len = y_run_rows(a.ny, R);
         const int nseg = (a.ny + b.seg_len - 1) / b.seg_len;
         k_sweep_y<T, F, P, E><<<dim3((unsigned) (ncolblk * nseg)), dim3(kYBlock), 0, s>>>(b, ncolblk);
         check_launch("sweep_y");
