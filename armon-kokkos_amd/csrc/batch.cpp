@@ -596,7 +596,7 @@ ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out)
                                  : batch_resident_blocks<float>(ARMON_AXIS_X, b->var);
         const int ry = b->fp64() ? batch_resident_blocks<double>(ARMON_AXIS_Y, b->var)
                                  : batch_resident_blocks<float>(ARMON_AXIS_Y, b->var);
-        const int reps = 3;
+        const int reps = 5;
 
         // X: persistent grids of 1, 2, 4 resident rounds, or one block per tile
         double best_x = 1e300;
