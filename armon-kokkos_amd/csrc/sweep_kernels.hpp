@@ -6,12 +6,17 @@
 // fluxes -> Lagrangian cell update -> first/second-order advection fluxes -> Euler
 // projection, reading the state once and writing it once (out of place, ping-pong buffers).
 //
-//  * X sweep (k_sweep_x): one 256-thread block per row tile. Each thread owns one cell of
-//    the tile (outputs plus an R-cell halo on each side, R = ghost layers read); the stage
-//    results are exchanged through LDS between barriers. Loads and stores are contiguous.
+//  * X sweep (k_sweep_x): one 256-thread block per tile of a stream of padded rows. Each
+//    thread owns one cell of the tile (outputs plus an R-cell halo on each side, R = ghost
+//    layers read); the stage results are exchanged through LDS between barriers. Loads and
+//    stores are contiguous.
 //  * Y sweep (k_sweep_y): one thread per column, marching down a run of rows. Every
 //    stage's stencil neighbours along y live in the thread's own registers (a sliding
 //    window), so no LDS and no barriers; consecutive lanes touch consecutive columns.
+//  * Both number their blocks XCD by XCD (xcd_block) so that each XCD's L2 serves a
+//    contiguous part of the arrays.
+//  * The batched ensemble (k_batch_x, k_batch_x1, k_batch_y) runs the same tile and run
+//    bodies over a work list that spans many domains.
 //
 // Ghost cells are synthesised in registers: layers within the stencil width mirror the real
 // cells with the side's velocity factors (halo_exchange.cpp:9-36); deeper layers keep their
