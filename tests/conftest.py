@@ -36,3 +36,21 @@ def po():
     import pyoracle
 
     return pyoracle
+
+
+@pytest.fixture(scope="session")
+def kats():
+    """Known answers of the reference's own tests (tests/golden/reference_kats.json)."""
+    import json
+
+    with open(os.path.join(ROOT, "tests", "golden", "reference_kats.json")) as f:
+        return json.load(f)
+
+
+def reference_cycle_counts():
+    """(test, cycles) pairs of the reference's NaN-check suite, for parametrize."""
+    import json
+
+    with open(os.path.join(ROOT, "tests", "golden", "reference_kats.json")) as f:
+        counts = json.load(f)["cycle_counts"]
+    return [(k, v) for k, v in counts.items() if k != "source"]

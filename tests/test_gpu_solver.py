@@ -11,6 +11,8 @@ import sys
 import numpy as np
 import pytest
 
+from conftest import reference_cycle_counts
+
 pytestmark = pytest.mark.gpu
 
 STATE = ("rho", "umat", "vmat", "Emat", "pmat", "cmat")
@@ -63,8 +65,7 @@ REF = dict(scheme=1, projection=2, limiter=1, axis_splitting=0, nb_ghosts=5, nx=
 
 
 @pytest.mark.parametrize("engine", ["fused", "staged"])
-@pytest.mark.parametrize("test,cycles", [("Sod", 45), ("Sod_y", 45), ("Sod_circ", 43), ("Bizarrium", 76),
-                                         ("Sedov", 568)])
+@pytest.mark.parametrize("test,cycles", reference_cycle_counts())
 def test_reference_cycle_counts(armon, po, oracle, engine, test, cycles):
     """tests/armon_2D_tests.cpp:238-242 with the reference parameters (reference.cpp:33-64)."""
     assert check(armon, po, oracle, test, ENGINES[engine], **REF) == cycles

@@ -16,6 +16,8 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
+from conftest import reference_cycle_counts
+
 
 def default_params(po, oracle):
     # armon_2D_tests.cpp:61-80
@@ -26,7 +28,7 @@ def default_params(po, oracle):
 
 # ---- indexing KATs: armon_2D_tests.cpp:83-189 --------------------------------------------
 
-def test_indexing_domains(po, oracle):
+def test_indexing_domains(po, oracle, kats):
     p = default_params(po, oracle)
     assert oracle.o_params_check(C.byref(p)) == 1
     P = C.byref(p)
@@ -34,35 +36,31 @@ def test_indexing_domains(po, oracle):
         "real": oracle.o_real_domain(P), "fluxes": oracle.o_domain_fluxes(P),
         "advection": oracle.o_domain_advection(P), "complete": oracle.o_complete_domain(P),
     }
-    assert doms["real"].as_tuple() == (555, 110, 995, 0, 1, 99)
-    assert doms["fluxes"].as_tuple() == (555, 110, 995, -2, 1, 102)
-    assert doms["advection"].as_tuple() == (555, 110, 995, 0, 1, 100)
-    assert doms["complete"].as_tuple() == (5, 110, 1545, -5, 1, 104)
-    exp = {"real": (500, 100, 5, 555, 1094), "fluxes": (525, 105, 5, 553, 1097),
-           "advection": (505, 101, 5, 555, 1095), "complete": (1650, 110, 15, 0, 1649)}
-    for k, (length, rl, cl, b, e) in exp.items():
+    for k, exp in kats["indexing"]["domains"].items():
         d = C.byref(doms[k])
-        assert oracle.o_dr_length(d) == length, k
-        assert oracle.o_dr_row_length(d) == rl, k
-        assert oracle.o_dr_col_length(d) == cl, k
-        assert oracle.o_dr_begin(d) == b, k
-        assert oracle.o_dr_end(d) == e, k
+        assert doms[k].as_tuple() == tuple(exp["range"]), k
+        assert oracle.o_dr_length(d) == exp["length"], k
+        assert oracle.o_dr_row_length(d) == exp["row_length"], k
+        assert oracle.o_dr_col_length(d) == exp["col_length"], k
+        assert oracle.o_dr_begin(d) == exp["begin"], k
+        assert oracle.o_dr_end(d) == exp["end"], k
 
 
-def test_indexing_ranges(po, oracle):
+def test_indexing_ranges(po, oracle, kats):
+    k1, k2 = kats["indexing"]["iter1D_real"], kats["indexing"]["iter2D_real"]
     p = default_params(po, oracle)
     d = oracle.o_real_domain(C.byref(p))
     r, i1 = po.ORange(), po.OInnerRange1D()
     oracle.o_dr_iter1d(C.byref(d), C.byref(r), C.byref(i1))
-    assert (r.start, r.end) == (0, 540)
-    assert i1.start == 555
+    assert [r.start, r.end] == k1["range"]
+    assert i1.start == k1["start"]
     assert oracle.o_scale_index_1d(C.byref(i1), r.start) == oracle.o_dr_begin(C.byref(d))
     assert oracle.o_scale_index_1d(C.byref(i1), r.end - 1) == oracle.o_dr_end(C.byref(d))
 
     i2 = po.OInnerRange2D()
     oracle.o_dr_iter2d(C.byref(d), C.byref(r), C.byref(i2))
-    assert (r.start, r.end) == (0, 500)
-    assert (i2.main_range_start, i2.main_range_step, i2.row_range_start, i2.row_range_length) == (555, 110, 0, 100)
+    assert [r.start, r.end] == k2["range"]
+    assert [i2.main_range_start, i2.main_range_step, i2.row_range_start, i2.row_range_length] == k2["inner"]
     s = lambda k: oracle.o_scale_index_2d(C.byref(i2), k)  # noqa: E731
     assert s(r.start) == oracle.o_dr_begin(C.byref(d))
     assert s(r.end - 1) == oracle.o_dr_end(C.byref(d))
@@ -70,27 +68,24 @@ def test_indexing_ranges(po, oracle):
     assert s(r.start + 200) == oracle.o_dr_begin(C.byref(d)) + 2 * p.row_length
 
 
-def test_indexing_boundary_conditions(po, oracle):
+def test_indexing_boundary_conditions(po, oracle, kats):
     p = default_params(po, oracle)
     disp = C.c_int()
-    exp = {po.SIDE_BOTTOM: (445, 544, 100, 110), po.SIDE_LEFT: (554, 994, 5, 1),
-           po.SIDE_RIGHT: (655, 1095, 5, -1), po.SIDE_TOP: (1105, 1204, 100, -110)}
-    for side, (b, e, n, dsp) in exp.items():
-        d = oracle.o_boundary_conditions_domain(C.byref(p), side, C.byref(disp))
-        assert (oracle.o_dr_begin(C.byref(d)), oracle.o_dr_end(C.byref(d)), oracle.o_dr_length(C.byref(d)),
-                disp.value) == (b, e, n, dsp), side
-    assert oracle.o_index_1D(C.byref(p), -1, 0) == 554
-    assert oracle.o_index_1D(C.byref(p), 0, -1) == 445
-    assert oracle.o_index_1D(C.byref(p), p.nx, 0) == 655
-    assert oracle.o_index_1D(C.byref(p), 0, p.ny) == 1105
+    sides = {"Bottom": po.SIDE_BOTTOM, "Left": po.SIDE_LEFT, "Right": po.SIDE_RIGHT, "Top": po.SIDE_TOP}
+    for name, exp in kats["indexing"]["boundary_conditions"].items():
+        d = oracle.o_boundary_conditions_domain(C.byref(p), sides[name], C.byref(disp))
+        got = (oracle.o_dr_begin(C.byref(d)), oracle.o_dr_end(C.byref(d)), oracle.o_dr_length(C.byref(d)), disp.value)
+        assert got == (exp["begin"], exp["end"], exp["length"], exp["disp"]), name
+    for i, j, idx in kats["indexing"]["index_1D"]:
+        assert oracle.o_index_1D(C.byref(p), i, j) == idx
 
 
-def test_memory_alignment(po, oracle):
+def test_memory_alignment(po, oracle, kats):
     # armon_2D_tests.cpp:192-202
     p = default_params(po, oracle)
     d = po.Data(oracle, p.nb_cells)
     for f, _ in po.OData._fields_:
-        assert C.cast(getattr(d.c, f), C.c_void_p).value % 64 == 0, f
+        assert C.cast(getattr(d.c, f), C.c_void_p).value % kats["indexing"]["alignment_bytes"] == 0, f
 
 
 # ---- whole-run KATs: armon_2D_tests.cpp:205-275 --------------------------------------------
@@ -100,8 +95,7 @@ FE_INVALID = 0x01
 FE_ALL_EXCEPT = 0x3D
 
 
-@pytest.mark.parametrize("test,cycles", [("Sod", 45), ("Sod_y", 45), ("Sod_circ", 43), ("Bizarrium", 76),
-                                         ("Sedov", 568)])
+@pytest.mark.parametrize("test,cycles", reference_cycle_counts())
 def test_nan_check_cycle_counts(po, oracle, test, cycles):
     """Reference params (reference.cpp:33-64) to the test's end time: the reference's cycle
     counts, and no invalid floating-point operation (the reference traps FE_INVALID)."""
