@@ -1,4 +1,4 @@
-// XCD-aware block numbering, shared by the fused sweeps and the per-stage kernels.
+// XCD-aware block numbering for the fused sweep kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
