@@ -7,9 +7,11 @@
 - Bizarrium 1024x1024 for 200 cycles (minmod) and 500 steps (superbee): identical.
 - The 1D tubes at full size (Bizarrium 4096x4096 for 500 steps, Sod 8192x8192 for 200):
   every row stays equal to the first.
-- Size-independent properties at 8192x8192: mass and energy conserved to 1e-12 over 100
-  cycles (FP64 fused), and the FP32-mixed form of circular Sod after 200 cycles within the
-  density L1 bound of 1e-5 against the FP64 engine."""
+- A size-independent property at 8192x8192: mass and energy conserved to 1e-12 over 100
+  cycles (FP64 fused).
+
+Whole runs whose oracle side takes minutes (Sedov 1024^2 to t = 1, Bizarrium 4096^2, circular
+Sod 8192^2 with its FP32-mixed form) are in test_gpu_golden_runs.py."""
 import ctypes as C
 
 import numpy as np
@@ -102,20 +104,3 @@ def test_conservation_8192(armon):
     assert abs(m1.value - m0.value) <= 1e-12 * abs(m0.value)
     assert abs(e1.value - e0.value) <= 1e-12 * abs(e0.value)
     s.close()
-
-
-def test_fp32_mixed_sod_circ_8192(armon):
-    cfg = dict(BENCH, nx=8192, ny=8192, max_cycles=200, max_time=1e9)
-    print("case Sod_circ 8192^2 200 cycles FP64 / FP32-mixed", flush=True)
-    r64, s64 = armon.run(armon.make_params("Sod_circ", engine=1, **cfg))
-    r32, s32 = armon.run(armon.make_params("Sod_circ", engine=1, precision=1, **cfg))
-    assert r64.status == 0 and r32.status == 0 and r64.cycles == r32.cycles == 200
-    mask = s64.read("domain_mask") == 1
-    errs = {}
-    for f in ("rho", "umat", "vmat", "Emat", "pmat"):
-        a, b = s32.read(f)[mask], s64.read(f)[mask]
-        errs[f] = (float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))), float(np.max(np.abs(a - b))))
-    print("FP32-mixed vs FP64 (rel L1, Linf):", errs, flush=True)
-    assert errs["rho"][0] <= 1e-5
-    s64.close()
-    s32.close()
