@@ -700,6 +700,13 @@ ARMON_EXPORT int armon_params_init(ArmonParams* p)
         p->domain_origin[1] = p->test == 4 ? -1 : 0;
     }
     if (p->stencil_width == -1) p->stencil_width = armon_min_ghosts(p);
+    // The indexing fields are ints (as in the reference's Params): refuse a domain whose array
+    // length, ghosts included, does not fit one.
+    if ((int64_t) (2 * (int64_t) p->nb_ghosts + p->nx) * (2 * (int64_t) p->nb_ghosts + p->ny) > INT32_MAX) {
+        fprintf(stderr, "ERROR: the domain (%d x %d cells, %d ghost layers) has more than 2^31 - 1 cells\n", p->nx,
+                p->ny, p->nb_ghosts);
+        return 1;
+    }
     init_indexing(p);
     armon_update_axis(p, p->current_axis);
     return 0;

@@ -129,7 +129,9 @@ typedef struct ArmonParams {
 } ArmonParams;
 
 ARMON_EXPORT void armon_params_default(ArmonParams* p);              /* parameters.h:34-97 defaults */
-ARMON_EXPORT int armon_params_init(ArmonParams* p);                  /* parameters.cpp:21-67 */
+/* parameters.cpp:21-67. Returns 1 (and leaves the indexing fields unset) for an unknown test or
+   a domain of 2^31 cells or more, ghosts included, whose int indexing would wrap around. */
+ARMON_EXPORT int armon_params_init(ArmonParams* p);
 ARMON_EXPORT int armon_params_check(const ArmonParams* p);           /* parameters.cpp:70-109: 1 if valid */
 ARMON_EXPORT int armon_min_ghosts(const ArmonParams* p);             /* parameters.cpp:11-18 */
 ARMON_EXPORT void armon_update_axis(ArmonParams* p, int axis);       /* parameters.cpp:195-210 */

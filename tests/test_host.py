@@ -140,6 +140,16 @@ def test_params_check_errors(armon):
     assert lib.armon_params_check(C.byref(p)) == 1
 
 
+def test_params_init_rejects_int_overflow(armon):
+    """The indexing fields are ints: a domain of 2^31 cells or more (ghosts included) is refused
+    by armon_params_init instead of wrapping around."""
+    armon.make_params("Sod", nx=46332, ny=46332, nb_ghosts=4)          # 46340^2 < 2^31
+    with pytest.raises(armon.ArmonError):
+        armon.make_params("Sod", nx=46333, ny=46333, nb_ghosts=4)      # 46341^2 >= 2^31
+    with pytest.raises(armon.ArmonError):
+        armon.make_params("Sod", nx=1 << 20, ny=1 << 12, nb_ghosts=4)
+
+
 def test_sedov_test_option(armon):
     """test_cases.h:118-120: r = hypot(dx, dy)/sqrt(2), i.e. dx for square cells."""
     lib = armon.load()
