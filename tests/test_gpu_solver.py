@@ -72,6 +72,26 @@ def test_reference_cycle_counts(armon, po, oracle, engine, test, cycles):
 
 
 @pytest.mark.parametrize("engine", ["fused", "staged"])
+@pytest.mark.parametrize("test,nx,ny", [("Sedov", 4, 300), ("Sedov", 300, 4), ("Sod_circ", 257, 5),
+                                        ("Sod_y", 5, 257), ("Bizarrium", 1023, 4)])
+def test_thin_and_ragged_domains(armon, po, oracle, engine, test, nx, ny):
+    """Domains as thin as the stencil's read extent (4 cells for GAD + 2nd-order remap) and sizes
+    that fill no tile or column group evenly: X tiles spanning several rows of the padded-row
+    stream, Y runs over partial groups of 64 columns."""
+    cfg = dict(scheme=1, projection=2, limiter=1, axis_splitting=0, nb_ghosts=4, nx=nx, ny=ny, max_cycles=60,
+               verbose=5)
+    assert check(armon, po, oracle, test, ENGINES[engine], **cfg) > 0
+
+
+@pytest.mark.parametrize("test,nx,ny", [("Sod", 1, 40), ("Sod_y", 40, 1), ("Sod", 1, 1)])
+def test_single_cell_wide_domains_staged(armon, po, oracle, test, nx, ny):
+    """One-cell-wide domains, below the fused engine's minimum width, through the staged engine:
+    every stage's range degenerates to one row or column."""
+    cfg = dict(scheme=0, projection=1, axis_splitting=0, nb_ghosts=2, nx=nx, ny=ny, max_cycles=30, verbose=5)
+    assert check(armon, po, oracle, test, ENGINES["staged"], **cfg) > 0
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
 def test_sod_first_order(armon, po, oracle, engine):
     """Sod 100x100, Godunov + first-order remap, CFL 0.6, to t = 0.2, default 3 ghosts and a
     stencil width of 1: the second ghost layer is read but never refilled (stale values)."""

@@ -316,3 +316,36 @@ def test_conservation_vars(armon, po, oracle):
     ce = [cm[j] * c.host["Emat"][i] for j, i in enumerate(idx)]
     assert abs(m.value - fsum(cm)) <= 2e-16 * abs(fsum(cm))
     assert abs(e.value - fsum(ce)) <= 2e-16 * abs(fsum(ce))
+
+
+def test_empty_ranges_change_nothing(armon):
+    """An empty iteration range (start == end) does nothing for every entry point, as a Kokkos
+    parallel_for over an empty RangePolicy, and the reductions return their identities: dt_CFL
+    the largest finite double (Kokkos' Min identity), conservation_vars zeros."""
+    lib = armon.load()
+    p = domain(armon)
+    names = sorted(set(ADV) | {"pstar", "u", "pmat", "cmat", "gmat", "domain_mask"})
+    c = Case(armon, p.nb_cells, names)
+    r, ir = armon.Range(), armon.InnerRange2D()
+    lib.armon_domain_iter2D(C.byref(lib.armon_real_domain(C.byref(p))), C.byref(r), C.byref(ir))
+    r.end = r.start
+    R, IR = C.byref(r), C.byref(ir)
+    v = lambda *ks: [c.v(k) for k in ks]
+    dt = 0.0071
+    lib.acoustic(R, IR, p.s, *v("ustar", "pstar", "rho", "u", "pmat", "cmat"))
+    lib.acoustic_GAD(R, IR, p.s, dt, p.dx, *v("ustar", "pstar", "rho", "u", "pmat", "cmat"), 1)
+    lib.perfect_gas_EOS(R, IR, 1.4, *v("rho", "Emat", "umat", "vmat", "pmat", "cmat", "gmat"))
+    lib.bizarrium_EOS(R, IR, *v("rho", "umat", "vmat", "Emat", "pmat", "cmat", "gmat"))
+    lib.cell_update(R, IR, p.s, p.dx, dt, *v("ustar", "pstar", "rho", "u", "Emat"))
+    lib.advection_first_order(R, IR, p.s, dt, *v(*ADV))
+    lib.advection_second_order(R, IR, p.s, p.dx, dt, *v(*ADV))
+    lib.euler_projection(R, IR, p.s, p.dx, dt, *v(*ADV))
+    r1, ir1 = armon.Range(), armon.InnerRange1D()
+    lib.armon_domain_iter1D(C.byref(lib.armon_real_domain(C.byref(p))), C.byref(r1), C.byref(ir1))
+    r1.end = r1.start
+    assert lib.dt_CFL(C.byref(r1), C.byref(ir1), 0.1, 0.1, *v("umat", "vmat", "cmat", "domain_mask")) == \
+        np.finfo(np.float64).max
+    m, e = C.c_double(1.0), C.c_double(1.0)
+    lib.conservation_vars(C.byref(r1), C.byref(ir1), p.dx, *v("rho", "Emat", "domain_mask"), C.byref(m), C.byref(e))
+    assert (m.value, e.value) == (0.0, 0.0)
+    c.compare(names)
