@@ -65,12 +65,15 @@ def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles, graph_c
     s.advance(steps)
     sync()
     elapsed = time.perf_counter() - t0
+    # per-kernel durations: `profile_cycles` more cycles launched one kernel at a time between
+    # HIP events on the solver stream, straight after the timed region (an idle device would
+    # first lower its clocks and the durations would not be those of the timed cycles)
+    kt = s.kernel_times(profile_cycles)
     r = armon.ArmonResult()
     st = lib.armon_solver_finish(s.h, C.byref(r))
-    if st != 0 or r.cycles != warmup + steps:
+    if st != 0 or r.cycles != warmup + steps + profile_cycles:
         raise RuntimeError(f"run failed: status {st}, cycles {r.cycles}, {armon.errors()}")
     rho = s.read("rho")
-    kt = s.kernel_times(profile_cycles)
     s.close()
     return elapsed, kt, rho
 
@@ -126,7 +129,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cells", type=int, default=8192)
     ap.add_argument("--cpu-cycles", type=int, default=40)
     ap.add_argument("--graph-chunk", type=int, default=16)
