@@ -164,3 +164,22 @@ def test_batch_fp32_mixed(armon):
         assert l1 <= 1e-5, (k, l1)
     b64.close()
     b32.close()
+
+
+def test_batch_fp32_mixed_independent_of_launch_configuration(armon):
+    """FP32-mixed results are not bit-comparable with the oracle, but they must not depend on how
+    the work is split: every run length and grid gives the same bits (each cell is computed by the
+    same expressions, contracted the same way, wherever its tile or run falls)."""
+    specs = small_specs(armon, 23, 6)
+    ref = None
+    for tuning in [(16, 0, 0), (64, 7, 5), (1000, 1, 1), (33, 300, 0)]:
+        print(f"case batch fp32 mixed seed 23 tuning {tuning}", flush=True)
+        b = run_batch(armon, specs, tuning=tuning, **dict(BASE, max_cycles=25, precision=1))
+        out = [b.read(k, f) for k in range(b.n) for f in STATE]
+        b.close()
+        if ref is None:
+            ref = out
+            continue
+        bad = [i for i, (a, o) in enumerate(zip(out, ref)) if not np.array_equal(a.view(np.uint64), o.view(np.uint64))]
+        assert not bad, (tuning, [(i // len(STATE), STATE[i % len(STATE)]) for i in bad[:8]])
+
