@@ -83,6 +83,7 @@ class ArmonParams(C.Structure):
         ("output_precision", C.c_int), ("output_file", C.c_char_p),
         ("engine", C.c_int), ("precision", C.c_int), ("graph_chunk", C.c_int),
         ("compare", C.c_int), ("comparison_tolerance", C.c_double),
+        ("y_run_rows", C.c_int),
     ]
 
 

@@ -612,7 +612,7 @@ ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out)
         // Y: run length x (one resident round of blocks | one block per 4 runs)
         double best_y = 1e300;
         int grid_y = 0, run_y = 256;
-        for (int run : {64, 128, 256, 512, 1024}) {
+        for (int run : {16, 32, 64, 128, 256, 512, 1024}) {
             build_items(b, run);
             for (int g : {ry, 0}) {
                 b->tuning.grid_y = g;

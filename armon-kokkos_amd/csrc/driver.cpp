@@ -394,7 +394,7 @@ SweepArgs<T> sweep_args(ArmonSolver* s, int axis, double factor, bool reduce, T*
     a.row_length = p.row_length;
     a.index_start = p.index_start;
     a.stencil_width = p.stencil_width;
-    a.seg_len = 0;
+    a.seg_len = p.y_run_rows > 0 ? p.y_run_rows : 0;   // 0: the launcher sizes the runs
     a.gamma = 1.4;
     a.dx = axis == ARMON_AXIS_X ? p.domain_size[0] / (double) p.nx : p.domain_size[1] / (double) p.ny;
     a.dx_pow2 = is_pow2(a.dx) ? 1 : 0;
@@ -673,6 +673,7 @@ ARMON_EXPORT void armon_params_default(ArmonParams* p)
     p->graph_chunk = 16;
     p->compare = 0;
     p->comparison_tolerance = 1e-13;
+    p->y_run_rows = 0;
 }
 
 // parameters.cpp:11-18

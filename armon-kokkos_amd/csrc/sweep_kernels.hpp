@@ -32,6 +32,7 @@
 #include "device_math.hpp"
 #include "xcd.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <stdexcept>
 #include <string>
@@ -452,6 +453,7 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     const int bid = xcd_block(blockIdx.x, gridDim.x);   // runs in segment-major order
     const int seg = bid / ncolblk;
     const int x = (bid - seg * ncolblk) * kYBlock + threadIdx.x;
+    if (x - (int) (threadIdx.x & 63) >= a.nx) return;   // a wave past the last column (no barriers follow)
     const int y0 = seg * a.seg_len;
     sweep_y_run<T, FLUX, PROJ, EOS>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
 }
@@ -567,21 +569,49 @@ inline int resident_blocks(const void* kernel, int block)
     return per_cu * cus;
 }
 
-// Rows per marching run of the Y sweep: about kYRun rows, with the number of runs per column
-// a multiple of the XCD count when the column is long enough, so that each XCD's contiguous
-// share of the blocks (xcd_block) is a set of whole row bands; never below 16 halo widths,
-// so that the 2-halo warm-up of a run stays small.
-constexpr int kYRun = 128;
-
-inline int y_run_rows(int ny, int halo)
+// SIMDs of the current device (4 per CU), looked up once per device.
+inline int device_simds()
 {
-    const int min_len = 16 * halo;
-    int nseg = (ny + kYRun - 1) / kYRun;
-    if (nseg >= kXcds) nseg = (nseg + kXcds - 1) / kXcds * kXcds;
-    int len = (ny + nseg - 1) / nseg;
-    if (len < min_len) len = min_len;
-    if (len > ny) len = ny;
-    return len > 0 ? len : 1;
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            return 1024;
+        cached[dev] = 4 * cus;
+    }
+    return cached[dev];
+}
+
+// Rows per marching run of the Y sweep, for a domain of nx x ny cells whose stencil reads
+// `halo` cells each side. A run of L output rows marches L + 2 halo rows (the warm-up), and a
+// wave of 64 columns marches one run, so shorter runs give more waves for more redundant work.
+// The rate of the march grows with the waves in flight as w / (w + SIMDs) (measured on MI355X
+// from 512^2 to 8192^2: the best L is 8 rows at 512^2, 16 at 1024^2, 32 at 2048^2, 32-64 at
+// 4096^2 and 64-128 at 8192^2), so L minimises (1 + 2 halo / L) (1 + SIMDs / waves) over the
+// powers of two from 8 to 128, the larger L on a tie. The number of runs per column is a
+// multiple of the XCD count when the column has at least that many, so that each XCD's
+// contiguous share of the blocks (xcd_block) is a set of whole row bands.
+inline int y_run_rows(int nx, int ny, int halo)
+{
+    if (ny <= 8) return ny > 0 ? ny : 1;
+    const double simds = (double) device_simds();
+    const double groups = (double) ((nx + 63) / 64);
+    int best = ny;
+    double best_cost = 0;
+    for (int L = 8; L <= 128; L *= 2) {
+        int nseg = (ny + L - 1) / L;
+        if (nseg >= kXcds) nseg = (nseg + kXcds - 1) / kXcds * kXcds;
+        const int len = (ny + nseg - 1) / nseg;
+        nseg = (ny + len - 1) / len;
+        const double cost = (1.0 + 2.0 * halo / len) * (1.0 + simds / (groups * nseg));
+        if (L == 8 || cost <= best_cost * (1.0 + 1e-3)) {
+            best = len;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 inline void check_launch(const char* what)
@@ -607,7 +637,7 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
         // per run); blocks are numbered row band by row band (segment-major).
         const int ncolblk = (a.nx + kYBlock - 1) / kYBlock;
         SweepArgs<T> b = a;
-        b.seg_len = y_run_rows(a.ny, R);
+        b.seg_len = a.seg_len > 0 ? std::min(a.seg_len, a.ny) : y_run_rows(a.nx, a.ny, R);
         const int nseg = (a.ny + b.seg_len - 1) / b.seg_len;
         k_sweep_y<T, F, P, E><<<dim3((unsigned) (ncolblk * nseg)), dim3(kYBlock), 0, s>>>(b, ncolblk);
         check_launch("sweep_y");

@@ -24,6 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
 WORKLOAD = dict(scheme=1, projection=2, limiter=1, axis_splitting=0, nb_ghosts=4, cfl=0.7)
+ACCURACY_FIELDS = ("rho", "umat", "vmat", "pmat", "Emat")
 
 
 def log(*a):
@@ -73,9 +74,10 @@ def measure_gpu(armon, nx, ny, steps, warmup, precision, profile_cycles, graph_c
     st = lib.armon_solver_finish(s.h, C.byref(r))
     if st != 0 or r.cycles != warmup + steps + profile_cycles:
         raise RuntimeError(f"run failed: status {st}, cycles {r.cycles}, {armon.errors()}")
-    rho = s.read("rho")
+    g, rl = p.nb_ghosts, p.row_length
+    state = {f: s.read(f).reshape(-1, rl)[g:g + ny, g:g + nx].copy() for f in ACCURACY_FIELDS}   # real cells
     s.close()
-    return elapsed, kt, rho
+    return elapsed, kt, state
 
 
 def measure_batch(armon, n_domains, seed, steps, warmup, precision):
@@ -155,7 +157,7 @@ def main():
     copy_gbps = armon.load().armon_copy_bandwidth(1 << 31, 20)
     log(f"[bench] device copy bandwidth (2 GiB buffers): {copy_gbps:.0f} GB/s")
     log(f"[bench] FP64 fused, Sedov {n}x{n}, warmup {args.warmup}, steps {args.steps}")
-    el64, kt64, rho64 = measure_gpu(armon, n, n, args.steps, args.warmup, 0, 10, args.graph_chunk)
+    el64, kt64, st64 = measure_gpu(armon, n, n, args.steps, args.warmup, 0, 10, args.graph_chunk)
     rate64 = cells * args.steps / el64
     log(f"[bench] FP64: {rate64 / 1e9:.3f} G cell-updates/s, {el64 / args.steps * 1e3:.3f} ms/cycle")
 
@@ -170,19 +172,23 @@ def main():
     fp32 = None
     if not args.no_fp32:
         log("[bench] FP32-mixed fused")
-        el32, kt32, rho32 = measure_gpu(armon, n, n, args.steps, args.warmup, 1, 10, args.graph_chunk)
+        el32, kt32, st32 = measure_gpu(armon, n, n, args.steps, args.warmup, 1, 10, args.graph_chunk)
         rate32 = cells * args.steps / el32
-        real = np.zeros_like(rho64, dtype=bool)
-        g_ = WORKLOAD["nb_ghosts"]
-        rl = n + 2 * g_
-        real.reshape(rl, rl)[g_:g_ + n, g_:g_ + n] = True
-        l1 = float(np.sum(np.abs(rho32[real] - rho64[real])) / np.sum(np.abs(rho64[real])))
-        linf = float(np.max(np.abs(rho32[real] - rho64[real])))
+        # relative L1 and absolute Linf error of each field against the FP64 run, same cycles
+        errors = {}
+        for f in ACCURACY_FIELDS:
+            d = np.abs(st32[f] - st64[f])
+            errors[f] = {"rel_L1": float(np.sum(d) / np.sum(np.abs(st64[f]))), "Linf": float(np.max(d))}
+        del st32, st64
+        l1, linf = errors["rho"]["rel_L1"], errors["rho"]["Linf"]
         fp32 = {"value": rate32, "ms_per_step": el32 / args.steps * 1e3, "speedup_vs_fp64": rate32 / rate64,
-                "rho_rel_L1_vs_fp64": l1, "rho_Linf_vs_fp64": linf,
+                "rho_rel_L1_vs_fp64": l1, "rho_Linf_vs_fp64": linf, "errors_vs_fp64": errors,
+                "cycles_compared": args.warmup + args.steps + 10,
                 "sweep_ms": {names[k]: kt32.total_ms[k] / max(kt32.launches[k], 1) for k in range(3)}}
         log(f"[bench] FP32-mixed: {rate32 / 1e9:.3f} G cell-updates/s, speedup {rate32 / rate64:.2f}, "
             f"rho L1 {l1:.2e}")
+        log("[bench] FP32-mixed vs FP64 (rel L1 / Linf): " +
+            ", ".join(f"{f} {e['rel_L1']:.2e} / {e['Linf']:.2e}" for f, e in errors.items()))
 
     batched = None
     if not args.no_batch:

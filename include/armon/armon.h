@@ -126,6 +126,7 @@ typedef struct ArmonParams {
     /* per-step comparison with reference files (parameters.h:95-97; staged engine only) */
     int compare;
     double comparison_tolerance;    /* relative, default 1e-13 */
+    int y_run_rows;                 /* fused Y sweep: rows per marching run; 0: chosen from the domain size */
 } ArmonParams;
 
 ARMON_EXPORT void armon_params_default(ArmonParams* p);              /* parameters.h:34-97 defaults */

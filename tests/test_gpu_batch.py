@@ -94,7 +94,7 @@ def test_batch_autotune(armon, po, oracle):
     print("case batch autotune", flush=True)
     b = run_batch(armon, specs, autotune=True, **dict(BASE, max_cycles=20))
     t = b.tuning()
-    assert t.y_run_rows in (64, 128, 256, 512, 1024) and t.ms_x > 0 and t.ms_y > 0
+    assert t.y_run_rows in (16, 32, 64, 128, 256, 512, 1024) and t.ms_x > 0 and t.ms_y > 0
     assert t.items_x > 0 and t.items_y > 0
     compare_batch(armon, po, oracle, b, specs)
     b.close()
