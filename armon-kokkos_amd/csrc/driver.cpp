@@ -394,7 +394,7 @@ SweepArgs<T> sweep_args(ArmonSolver* s, int axis, double factor, bool reduce, T*
     a.row_length = p.row_length;
     a.index_start = p.index_start;
     a.stencil_width = p.stencil_width;
-    a.seg_len = p.y_run_rows > 0 ? p.y_run_rows : 0;   // 0: the launcher sizes the runs
+    a.seg_len = p.y_run_rows;   // > 0: rows per run; 0: the launcher chooses
     a.gamma = 1.4;
     a.dx = axis == ARMON_AXIS_X ? p.domain_size[0] / (double) p.nx : p.domain_size[1] / (double) p.ny;
     a.dx_pow2 = is_pow2(a.dx) ? 1 : 0;
@@ -742,6 +742,10 @@ ARMON_EXPORT int armon_params_check(const ArmonParams* p)
     if (p->stencil_width > p->nb_ghosts) {
         fprintf(stderr, "ERROR: The stencil width cannot be bigger than the number of ghost cells: %d > %d\n",
                 p->stencil_width, p->nb_ghosts);
+        return 0;
+    }
+    if (p->y_run_rows < 0) {
+        fprintf(stderr, "ERROR: y_run_rows must be positive or 0 (automatic), got %d\n", p->y_run_rows);
         return 0;
     }
     if (p->engine != ARMON_ENGINE_STAGED && p->engine != ARMON_ENGINE_FUSED) {

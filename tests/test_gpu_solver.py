@@ -114,6 +114,33 @@ def test_fused_splittings(armon, po, oracle, split, test):
     check(armon, po, oracle, test, 1, **cfg)
 
 
+Y_SWEEP_CASES = [
+    ("Sedov", dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=64, ny=150)),
+    ("Sedov", dict(scheme=0, projection=1, limiter=1, nb_ghosts=4, nx=70, ny=64)),
+    ("Sedov", dict(scheme=0, projection=2, nb_ghosts=3, nx=40, ny=36)),          # stale 3rd ghost layer
+    ("Bizarrium", dict(scheme=1, projection=2, limiter=2, nb_ghosts=4, nx=96, ny=64)),
+    ("Sod_circ", dict(scheme=1, projection=2, limiter=1, axis_splitting=1, nb_ghosts=4, nx=50, ny=70)),
+    ("Sod_y", dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=5, ny=257)),
+    ("Sedov", dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=300, ny=4)),
+]
+
+
+@pytest.mark.parametrize("y_run_rows", [1, 3, 1000])
+@pytest.mark.parametrize("test,cfg", Y_SWEEP_CASES)
+def test_fused_y_run_lengths(armon, po, oracle, y_run_rows, test, cfg):
+    """The fused Y sweep with marching runs of any length (ArmonParams.y_run_rows) gives the
+    oracle's bits: the run length changes how the work is split, never the arithmetic."""
+    cfg = dict(cfg, max_cycles=30, verbose=5)
+    st, od, cycles, final_dt = oracle_run(po, oracle, test, **cfg)
+    assert st == 0
+    print(f"case {test} y_run_rows={y_run_rows} {cfg}", flush=True)
+    r, s = product_run(armon, test, 1, y_run_rows=y_run_rows, **cfg)
+    assert r.status == 0, armon.errors()
+    assert (r.cycles, r.final_dt) == (cycles, final_dt)
+    assert_same_state(s, od)
+    s.close()
+
+
 @pytest.mark.parametrize("limiter", [1, 2])
 def test_fused_bizarrium(armon, po, oracle, limiter):
     cfg = dict(scheme=1, projection=2, limiter=limiter, nb_ghosts=4, nx=96, ny=64, max_cycles=60, verbose=5)

@@ -138,6 +138,8 @@ def test_params_check_errors(armon):
     assert lib.armon_params_check(C.byref(p)) == 0
     p = armon.make_params("Sod", scheme=1, projection=2, nb_ghosts=4, engine=armon.ENGINE_FUSED)
     assert lib.armon_params_check(C.byref(p)) == 1
+    p = armon.make_params("Sod", scheme=1, projection=2, nb_ghosts=4, engine=armon.ENGINE_FUSED, y_run_rows=-1)
+    assert lib.armon_params_check(C.byref(p)) == 0
 
 
 def test_params_init_rejects_int_overflow(armon):
