@@ -222,7 +222,7 @@ def main():
         rate_cpu, threads, secs = cpu_baseline(n, n, args.cpu_cycles)
         cpu = {"value": rate_cpu, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                "sample": f"Sedov {n}x{n}, {args.cpu_cycles} cycles (15 per-stage loops per cycle, "
-                         f"-O3 -ffast-math OpenMP build of the oracle), {secs:.1f} s"}
+                         f"-O3 -ffast-math OpenMP build of the oracle, row-by-row 2D iteration), {secs:.1f} s"}
         log(f"[bench] CPU: {rate_cpu / 1e6:.1f} M cell-updates/s on {threads} threads")
 
     out = {
