@@ -1,5 +1,5 @@
 // Command-line driver: the flags of the reference (src/main_2D.cpp:9-38) plus the execution
-// options of this implementation (--engine, --arith, --graph-chunk).
+// options of this implementation (--engine, --arith, --graph-chunk, --y-run).
 #include "armon/armon.h"
 
 #include <chrono>
@@ -41,6 +41,7 @@ Options:
     --engine <engine>       'fused' (default: one kernel per axis sweep) or 'staged' (one launch per stage)
     --arith <mode>          'fp64' (default) or 'fp32mixed' (fused engine only)
     --graph-chunk N         Cycles per captured hipGraph launch (fused engine, even; 0 disables)
+    --y-run N               Rows per marching run of the fused Y sweep (0: chosen from the domain size)
     --batch N               Run a batched ensemble of N domains (64^2 to 1024^2 cells, Sod / Sod_circ /
                             Sedov, perturbed states) drawn from --seed S (default 2024), each to its own
                             end time or --cycle; --autotune 0-1 (default 1)
@@ -192,6 +193,10 @@ static bool parse_arguments(ArmonParams& p, BatchOptions& bo, int argc, char** a
         } else if (strcmp(a, "--graph-chunk") == 0) {
             if (!need()) return false;
             p.graph_chunk = (int) strtol(v, nullptr, 10);
+            i++;
+        } else if (strcmp(a, "--y-run") == 0) {
+            if (!need()) return false;
+            p.y_run_rows = (int) strtol(v, nullptr, 10);
             i++;
         } else if (strcmp(a, "--batch") == 0) {
             if (!need()) return false;
