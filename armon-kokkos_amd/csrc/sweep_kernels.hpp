@@ -52,6 +52,9 @@ constexpr int kYBlock = 256;  // Y sweep: columns per block
 #ifndef ARMON_Y_WAVES_PER_EU
 #define ARMON_Y_WAVES_PER_EU 0
 #endif
+#ifndef ARMON_Y_PREFETCH
+#define ARMON_Y_PREFETCH 1
+#endif
 constexpr int kYUnroll = ARMON_Y_UNROLL;
 #if ARMON_Y_WAVES_PER_EU > 0
 #define ARMON_Y_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ARMON_Y_WAVES_PER_EU)))
@@ -122,6 +125,61 @@ __device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k
     if (stale) {
         cell.p = a.stale_p[idx];
         cell.c = a.stale_c[idx];
+    } else {
+        cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);
+    }
+    return cell;
+}
+
+// load_cell in two parts, for the FP64 Y march, which loads a row's state one row before it
+// makes the cell: where cell `k` comes from (cell_source: the index it is read at, the mirror
+// side or staleness), then the cell from the loaded state (make_cell). The FP32-mixed march
+// keeps load_cell: split this way, its contracted FP32 arithmetic was no longer the same in
+// every unrolled copy of the march, and the result came to depend on the run length.
+struct CellSource {
+    int idx;
+    int mirror;
+    bool stale;
+};
+
+// STALE = false: the caller guarantees that the stencil width covers every ghost layer read
+// (|k| beyond the domain is at most the read extent), so no layer is stale.
+template <class T, int AXIS, bool STALE = true>
+__device__ __forceinline__ CellSource cell_source(const SweepArgs<T>& a, int q, int k, int n)
+{
+    int kk = k;
+    CellSource s{0, 0, false};
+    if (k < 0) {
+        if (!STALE || -k <= a.stencil_width) { kk = -k - 1; s.mirror = 1; }
+        else s.stale = true;
+    } else if (k >= n) {
+        if (!STALE || k - n + 1 <= a.stencil_width) { kk = 2 * n - 1 - k; s.mirror = 2; }
+        else s.stale = true;
+    }
+    s.idx = AXIS == 0 ? a.index_start + q * a.row_length + kk : a.index_start + kk * a.row_length + q;
+    return s;
+}
+
+// The cell from its loaded state (rho, u, v, E at s.idx): mirror factors, then EOS (or the
+// stale p and c).
+template <class T, int AXIS, int EOS, bool STALE = true>
+__device__ __forceinline__ Cell<T> make_cell(const SweepArgs<T>& a, const CellSource& s, T rho, T u, T v, T E)
+{
+    if (s.mirror == 1) {
+        u = u * (T) a.lo_u;
+        v = v * (T) a.lo_v;
+    } else if (s.mirror == 2) {
+        u = u * (T) a.hi_u;
+        v = v * (T) a.hi_v;
+    }
+    Cell<T> cell;
+    cell.rho = rho;
+    cell.E = E;
+    cell.un = AXIS == 0 ? u : v;
+    cell.ut = AXIS == 0 ? v : u;
+    if (STALE && s.stale) {
+        cell.p = a.stale_p[s.idx];
+        cell.c = a.stale_c[s.idx];
     } else {
         cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);
     }
@@ -325,8 +383,11 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
 }
 
 // One Y run: column x (lane_ok false for lanes past the domain), output rows [y0, y1),
-// with the cycle's dt. The whole wave must call it with the same y0, y1.
-template <class T, int FLUX, int PROJ, int EOS>
+// with the cycle's dt. The whole wave must call it with the same y0, y1. AHEAD: load each
+// row's state one row before it is used (FP64 single-domain sweep: its registers leave room
+// for it at 3 waves per SIMD); with STALE = false, when the stencil width covers the read
+// extent so that no ghost layer is stale, the march then loads nothing else.
+template <class T, int FLUX, int PROJ, int EOS, bool STALE, bool AHEAD>
 __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, bool lane_ok, int y0, int y1)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
@@ -355,9 +416,34 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
 
     double local = DBL_MAX;
 
+    // AHEAD: the state of the next row's cell is loaded one row ahead, so that its memory
+    // latency is covered by a whole row of arithmetic.
+    constexpr bool kAhead = AHEAD;
+    T nx_rho = T(1), nx_u = T(0), nx_v = T(0), nx_E = T(1);
+    auto fetch = [&](int k) {
+        if (lane_ok) {
+            const int idx = cell_source<T, 1, STALE>(a, x, k, a.ny).idx;
+            nx_rho = a.in_rho[idx];
+            nx_u = a.in_u[idx];
+            nx_v = a.in_v[idx];
+            nx_E = a.in_E[idx];
+        }
+    };
+    if constexpr (kAhead) fetch(y0 - R);
+
 #pragma unroll kYUnroll
     for (int n = y0 - R; n < y1 + R; ++n) {
-        const Cell<T> cn = lane_ok ? load_cell<T, 1, EOS>(a, x, n, a.ny) : benign_cell<T>();
+        Cell<T> cn;
+        if constexpr (kAhead) {
+            // the next load is issued after this row's cell is made: a stale layer's p, c loads
+            // are waited for there
+            cn = lane_ok ? make_cell<T, 1, EOS, STALE>(a, cell_source<T, 1, STALE>(a, x, n, a.ny), nx_rho, nx_u, nx_v,
+                                                       nx_E)
+                         : benign_cell<T>();
+            if (n + 1 < y1 + R) fetch(n + 1);
+        } else {
+            cn = lane_ok ? load_cell<T, 1, EOS>(a, x, n, a.ny) : benign_cell<T>();
+        }
         pre[0] = pre[1];
         pre[1] = pre[2];
         pre[2] = cn;
@@ -445,7 +531,7 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
     if (a.reduce) reduce_cfl(a.st, local);
 }
 
-template <class T, int FLUX, int PROJ, int EOS>
+template <class T, int FLUX, int PROJ, int EOS, bool STALE>
 __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs<T> a, int ncolblk)
 {
     if (!a.st->active) return;
@@ -455,7 +541,8 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     const int x = (bid - seg * ncolblk) * kYBlock + threadIdx.x;
     if (x - (int) (threadIdx.x & 63) >= a.nx) return;   // a wave past the last column (no barriers follow)
     const int y0 = seg * a.seg_len;
-    sweep_y_run<T, FLUX, PROJ, EOS>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
+    constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
+    sweep_y_run<T, FLUX, PROJ, EOS, STALE, ahead>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -543,7 +630,9 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY
         if (!st->active) continue;   // the same for the whole wave
         const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
         const int x = item.x0 + lane;
-        sweep_y_run<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0, item.y1);
+        // the per-item domain arguments leave no register room to load a row ahead
+        sweep_y_run<T, FLUX, PROJ, EOS, true, false>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0,
+                                                     item.y1);
     }
 }
 
@@ -639,7 +728,11 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
         SweepArgs<T> b = a;
         b.seg_len = a.seg_len > 0 ? std::min(a.seg_len, a.ny) : y_run_rows(a.nx, a.ny, R);
         const int nseg = (a.ny + b.seg_len - 1) / b.seg_len;
-        k_sweep_y<T, F, P, E><<<dim3((unsigned) (ncolblk * nseg)), dim3(kYBlock), 0, s>>>(b, ncolblk);
+        const dim3 grid((unsigned) (ncolblk * nseg));
+        if (a.stencil_width >= R)
+            k_sweep_y<T, F, P, E, false><<<grid, dim3(kYBlock), 0, s>>>(b, ncolblk);
+        else
+            k_sweep_y<T, F, P, E, true><<<grid, dim3(kYBlock), 0, s>>>(b, ncolblk);
         check_launch("sweep_y");
     }
 }
