@@ -134,8 +134,8 @@ __device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k
 // load_cell in two parts, for the FP64 Y march, which loads a row's state one row before it
 // makes the cell: where cell `k` comes from (cell_source: the index it is read at, the mirror
 // side or staleness), then the cell from the loaded state (make_cell). The FP32-mixed march
-// keeps load_cell: split this way, its contracted FP32 arithmetic was no longer the same in
-// every unrolled copy of the march, and the result came to depend on the run length.
+// keeps load_cell: at 90 VGPRs it runs 5 waves per SIMD, which already cover the load latency
+// (loading a row ahead gained about 1% there).
 struct CellSource {
     int idx;
     int mirror;
@@ -384,9 +384,9 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
 
 // One Y run: column x (lane_ok false for lanes past the domain), output rows [y0, y1),
 // with the cycle's dt. The whole wave must call it with the same y0, y1. AHEAD: load each
-// row's state one row before it is used (FP64 single-domain sweep: its registers leave room
-// for it at 3 waves per SIMD); with STALE = false, when the stencil width covers the read
-// extent so that no ghost layer is stale, the march then loads nothing else.
+// row's state one row before it is used (the FP64 sweeps); with STALE = false, when the
+// stencil width covers the read extent so that no ghost layer is stale, the march then loads
+// nothing else.
 template <class T, int FLUX, int PROJ, int EOS, bool STALE, bool AHEAD>
 __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, bool lane_ok, int y0, int y1)
 {
@@ -616,8 +616,12 @@ __global__ void __launch_bounds__(kTile) k_batch_x1(BatchArgs<T> b, const BatchI
     sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.slot, sm);
 }
 
-template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY
+// The row-ahead loads of the FP64 march need 179 VGPRs here (the per-item domain arguments
+// add to the single-domain sweep's 164), which would leave 2 waves per SIMD; capped at 3 waves,
+// the compiler keeps 7 dwords of item-invariant values in scratch instead (one 16-byte reload
+// per row), and the FP64 sweep is 4% faster than without the row-ahead loads.
+template <class T, int FLUX, int PROJ, int EOS, bool STALE>
+__global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)))
         k_batch_y(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
 {
     const int lane = threadIdx.x & 63;
@@ -630,9 +634,9 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY
         if (!st->active) continue;   // the same for the whole wave
         const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
         const int x = item.x0 + lane;
-        // the per-item domain arguments leave no register room to load a row ahead
-        sweep_y_run<T, FLUX, PROJ, EOS, true, false>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0,
-                                                     item.y1);
+        constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
+        sweep_y_run<T, FLUX, PROJ, EOS, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0,
+                                                      item.y1);
     }
 }
 
@@ -761,8 +765,14 @@ void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, in
         k_batch_x<T, F, P, E><<<dim3((unsigned) grid), dim3(kTile), 0, s>>>(b, static_cast<const BatchItemX*>(items),
                                                                              nitems);
     else
-        k_batch_y<T, F, P, E><<<dim3((unsigned) grid), dim3(kYBlock), 0, s>>>(b, static_cast<const BatchItemY*>(items),
-                                                                               nitems);
+    {
+        constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
+        const auto* y_items = static_cast<const BatchItemY*>(items);
+        if (b.stencil_width >= R)
+            k_batch_y<T, F, P, E, false><<<dim3((unsigned) grid), dim3(kYBlock), 0, s>>>(b, y_items, nitems);
+        else
+            k_batch_y<T, F, P, E, true><<<dim3((unsigned) grid), dim3(kYBlock), 0, s>>>(b, y_items, nitems);
+    }
     check_launch(axis == 0 ? "batch_x" : "batch_y");
 }
 
@@ -770,7 +780,7 @@ template <class T, int F, int P, int E>
 int batch_variant_resident(int axis)
 {
     return axis == 0 ? resident_blocks((const void*) k_batch_x<T, F, P, E>, kTile)
-                     : resident_blocks((const void*) k_batch_y<T, F, P, E>, kYBlock);
+                     : resident_blocks((const void*) k_batch_y<T, F, P, E, false>, kYBlock);
 }
 
 // Calls fn.template operator()<F, P, E>() for the variant's compile-time parameters.
