@@ -44,8 +44,9 @@ namespace sweep {
 constexpr int kTile = 256;    // X sweep: threads (cells) per tile
 constexpr int kYBlock = 256;  // Y sweep: columns per block
 
-// Y march: rows per unrolled loop body, and the minimum waves per SIMD the register
-// allocation must allow (0: the compiler's choice).
+// Y march: rows per unrolled loop body, the minimum waves per SIMD the register allocation
+// must allow (0: the compiler's choice), and whether the FP64 marches load each row's state
+// one row before they use it (1) or within the row (0).
 #ifndef ARMON_Y_UNROLL
 #define ARMON_Y_UNROLL 4
 #endif
