@@ -158,8 +158,32 @@ def _proto(lib):
     return lib
 
 
+# x86-64-v4 = AVX-512 F, BW, CD, DQ and VL (on top of x86-64-v3, which AVX-512 hosts have)
+_V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
+
+
+def host_has_x86_64_v4() -> bool:
+    """True when /proc/cpuinfo lists every AVX-512 subset of x86-64-v4."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return all(x in flags for x in _V4_FLAGS)
+    except OSError:
+        pass
+    return False
+
+
 def lib_path(strict: bool = True) -> str:
-    return os.path.join(_HERE, "lib", "liboracle.so" if strict else "liboracle_fast.so")
+    """The strict checker, or the throughput build for this host's CPU (x86-64-v4 when the
+    host has it and the library was built, else x86-64-v3)."""
+    if strict:
+        return os.path.join(_HERE, "lib", "liboracle.so")
+    v4 = os.path.join(_HERE, "lib", "liboracle_fast_v4.so")
+    if host_has_x86_64_v4() and os.path.exists(v4):
+        return v4
+    return os.path.join(_HERE, "lib", "liboracle_fast.so")
 
 
 def load(strict: bool = True):
