@@ -338,3 +338,18 @@ def test_stale_ghost_layers_keep_init_values(po, oracle):
     cm = _field(d, "cmat", p)
     assert np.all(pm[g:g + p.ny, g - 2] == 0.0) and np.all(cm[g:g + p.ny, g - 2] == 1.0)
     assert np.all(pm[g:g + p.ny, g + p.nx + 1] == 0.0) and np.all(cm[g:g + p.ny, g + p.nx + 1] == 1.0)
+
+
+def test_throughput_build_follows_the_host_cpu(po, monkeypatch):
+    """The CPU baseline loads the x86-64-v4 (AVX-512) build only on hosts whose CPU has every
+    AVX-512 subset of that level, else the x86-64-v3 build; the checker is always the strict one."""
+    import os
+
+    monkeypatch.setattr(po, "host_has_x86_64_v4", lambda: False)
+    assert os.path.basename(po.lib_path(False)) == "liboracle_fast.so"
+    monkeypatch.setattr(po, "host_has_x86_64_v4", lambda: True)
+    v4 = os.path.basename(po.lib_path(False))
+    assert v4 == ("liboracle_fast_v4.so" if os.path.exists(os.path.join(os.path.dirname(po.__file__), "lib",
+                                                                       "liboracle_fast_v4.so"))
+                  else "liboracle_fast.so")
+    assert os.path.basename(po.lib_path(True)) == "liboracle.so"
