@@ -84,6 +84,7 @@ class ArmonParams(C.Structure):
         ("engine", C.c_int), ("precision", C.c_int), ("graph_chunk", C.c_int),
         ("compare", C.c_int), ("comparison_tolerance", C.c_double),
         ("y_run_rows", C.c_int),
+        ("bc_override", C.c_int), ("bc_factors", (C.c_double * 2) * 4),
     ]
 
 
@@ -317,6 +318,10 @@ def make_params(test="Sod", **kw) -> ArmonParams:
     for k, v in kw.items():
         if k in ("domain_size", "domain_origin"):
             getattr(p, k)[0], getattr(p, k)[1] = v
+        elif k == "bc_factors":   # ((u, v) for Left, Right, Top, Bottom): overrides the test's
+            for side, (uf, vf) in enumerate(v):
+                p.bc_factors[side][0], p.bc_factors[side][1] = uf, vf
+            p.bc_override = 1
         elif k == "output_file":
             p.output_file = v.encode() if isinstance(v, str) else v
         else:

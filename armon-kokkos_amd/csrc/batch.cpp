@@ -487,7 +487,7 @@ ARMON_EXPORT ArmonBatch* armon_batch_create(const ArmonParams* common, int n, co
             }
             D.cfl_dx = D.dx[0];
             D.cfl_dy = D.dx[1];
-            for (int k = 0; k < 4; k++) armon_test_boundary_condition(q.test, sides[k], &D.bc[k][0], &D.bc[k][1]);
+            for (int k = 0; k < 4; k++) boundary_factors(q, sides[k], &D.bc[k][0], &D.bc[k][1]);
             D.x_slots = q.nx + 2 * read_extent(b->var.flux, b->var.proj);
             D.x_inv_slots = 1.0f / (float) D.x_slots;
             cfg[d] = TimeStepConfig{q.cfl, q.Dt, q.max_time, q.max_cycles, q.cst_dt, q.dt_on_even_cycles};

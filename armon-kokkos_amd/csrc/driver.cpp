@@ -223,7 +223,7 @@ void staged_boundary_conditions(ArmonSolver* s)
         int disp = 0;
         DomainRange d = armon_boundary_conditions_domain(&p, sides[k], &disp);
         double uf, vf;
-        armon_test_boundary_condition(p.test, sides[k], &uf, &vf);
+        boundary_factors(p, sides[k], &uf, &vf);
         Range range;
         InnerRange1D inner;
         armon_domain_direct_iter1D(&d, &range, &inner);
@@ -401,8 +401,8 @@ SweepArgs<T> sweep_args(ArmonSolver* s, int axis, double factor, bool reduce, T*
     a.inv_dx = a.dx_pow2 ? 1.0 / a.dx : 0.0;
     a.dt_factor = factor;
     double lu, lv, hu, hv;
-    armon_test_boundary_condition(p.test, axis == ARMON_AXIS_X ? ARMON_SIDE_LEFT : ARMON_SIDE_BOTTOM, &lu, &lv);
-    armon_test_boundary_condition(p.test, axis == ARMON_AXIS_X ? ARMON_SIDE_RIGHT : ARMON_SIDE_TOP, &hu, &hv);
+    boundary_factors(p, axis == ARMON_AXIS_X ? ARMON_SIDE_LEFT : ARMON_SIDE_BOTTOM, &lu, &lv);
+    boundary_factors(p, axis == ARMON_AXIS_X ? ARMON_SIDE_RIGHT : ARMON_SIDE_TOP, &hu, &hv);
     a.lo_u = lu; a.lo_v = lv; a.hi_u = hu; a.hi_v = hv;
     a.reduce = reduce ? 1 : 0;
     a.cfl_dx = p.domain_size[0] / (double) p.nx;

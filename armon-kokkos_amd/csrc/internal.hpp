@@ -38,6 +38,18 @@ inline void hip_check(hipError_t err, const char* what)
 
 hipStream_t stream();
 
+// The boundary velocity factors of `side` (ARMON_SIDE_*): the caller's when p.bc_override is
+// set, else the test's (armon_test_boundary_condition).
+inline void boundary_factors(const ArmonParams& p, int side, double* uf, double* vf)
+{
+    if (p.bc_override) {
+        *uf = p.bc_factors[side][0];
+        *vf = p.bc_factors[side][1];
+    } else {
+        armon_test_boundary_condition(p.test, side, uf, vf);
+    }
+}
+
 // Library-owned device scratch (reduction partials and scalars), one per device,
 // allocated on first use and kept until the process ends.
 struct Scratch {

@@ -127,6 +127,12 @@ typedef struct ArmonParams {
     int compare;
     double comparison_tolerance;    /* relative, default 1e-13 */
     int y_run_rows;                 /* fused Y sweep: rows per marching run; 0: chosen from the domain size */
+    /* boundary velocity factors (u, v) per side (ARMON_SIDE_*), used instead of the test's
+     * (test_cases.h:36-38 and the per-test overrides) when bc_override is set: a configuration
+     * outside the reference's tests, such as reflective walls around a Sedov blast at a corner.
+     * Both engines and the batched ensemble's common parameters honour it. */
+    int bc_override;
+    double bc_factors[4][2];
 } ArmonParams;
 
 ARMON_EXPORT void armon_params_default(ArmonParams* p);              /* parameters.h:34-97 defaults */

@@ -992,7 +992,12 @@ static void boundary_conditions_side(const OParams* p, OData* d, int side)
     int disp = 0;
     ODomainRange dr = o_boundary_conditions_domain(p, side, &disp);
     double uf, vf;
-    o_test_bc(p->test, side, &uf, &vf);
+    if (p->bc_override) {
+        uf = p->bc_factors[side][0];
+        vf = p->bc_factors[side][1];
+    } else {
+        o_test_bc(p->test, side, &uf, &vf);
+    }
     ORange r; OInnerRange1D ir;
     o_dr_direct_iter1d(&dr, &r, &ir);
     o_boundary_conditions(&r, &ir, p->stencil_width, disp, uf, vf,

@@ -158,6 +158,11 @@ typedef struct {
     int max_cycles;
     double max_time;
     int verbose;
+    /* boundary velocity factors (u, v) per side (O_SIDE_*), used instead of the test's
+     * (test_cases.h) when bc_override is set: a configuration outside the reference's tests,
+     * such as reflective walls around a Sedov blast at a corner */
+    int bc_override;
+    double bc_factors[4][2];
 } OParams;
 
 void   o_params_defaults(OParams* p);                 /* parameters.h:34-97 member initialisers */

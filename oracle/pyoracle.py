@@ -61,6 +61,7 @@ class OParams(C.Structure):
         ("cycle", C.c_int), ("cycle_dt", C.c_double), ("current_cycle_dt", C.c_double),
         ("next_cycle_dt", C.c_double), ("time", C.c_double),
         ("max_cycles", C.c_int), ("max_time", C.c_double), ("verbose", C.c_int),
+        ("bc_override", C.c_int), ("bc_factors", (C.c_double * 2) * 4),
     ]
 
 
@@ -211,6 +212,10 @@ def make_params(lib, test="Sod", **kw) -> OParams:
     for k, v in kw.items():
         if k in ("domain_size", "domain_origin"):
             getattr(p, k)[0], getattr(p, k)[1] = v
+        elif k == "bc_factors":   # ((u, v) for Left, Right, Top, Bottom): overrides the test's
+            for side, (uf, vf) in enumerate(v):
+                p.bc_factors[side][0], p.bc_factors[side][1] = uf, vf
+            p.bc_override = 1
         else:
             setattr(p, k, v)
     lib.o_params_init(C.byref(p))
@@ -247,14 +252,19 @@ class Data:
             pass
 
 
-def run(lib, p: OParams, debug_indexes: bool = False, states: "OTestState | None" = None):
+def run(lib, p: OParams, debug_indexes: bool = False, states: "OTestState | None" = None,
+        init_fields: "dict | None" = None):
     """init_test + time_loop (armon_2D.cpp:389-407). Returns (status, data, cycles, final_dt, loop_time).
-    ``states`` replaces the test's reference high/low states (a batched-ensemble domain)."""
+    ``states`` replaces the test's reference high/low states (a batched-ensemble domain);
+    ``init_fields`` (field name -> array of nb_cells values) overwrites fields after init_test,
+    for initial conditions outside the reference's tests."""
     d = Data(lib, p.nb_cells)
     if states is not None:
         lib.o_init_test_params_states(C.byref(p), C.byref(d.c), C.byref(states))
     else:
         lib.o_init_test_params(C.byref(p), C.byref(d.c), bool(debug_indexes))
+    for name, values in (init_fields or {}).items():
+        d.array(name)[:] = np.asarray(values, dtype=np.float64).reshape(-1)
     loop_time, final_dt, cycles = C.c_double(), C.c_double(), C.c_int()
     st = lib.o_time_loop(C.byref(p), C.byref(d.c), C.byref(loop_time), C.byref(final_dt), C.byref(cycles))
     return st, d, cycles.value, final_dt.value, loop_time.value

@@ -251,3 +251,27 @@ def test_fp32_mixed_accuracy(armon, test):
     print(test, {k: f"L1={v[0]:.2e} Linf={v[1]:.2e}" for k, v in out.items()}, file=sys.stderr)
     assert out["rho"][0] <= 1e-5
     assert p.nx == 256
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_sedov_corner_overrides(armon, po, oracle, engine):
+    """A configuration outside the reference's tests, given through the explicit overrides (origin,
+    size, per-side boundary factors, initial fields): the Sedov blast in the 2x2 cells at the
+    origin corner of [0, 1]^2, reflective walls, background internal energy 1e-5 (GAD + minmod,
+    second-order remap, CFL 0.7), 256x256 for 300 cycles, bit for bit against the oracle given the
+    same overrides."""
+    from conftest import SEDOV_CORNER, sedov_corner_fields
+
+    cfg = dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=256, ny=256, cfl=0.7, max_cycles=300,
+               max_time=1.0, verbose=5, **SEDOV_CORNER)
+    p = po.make_params(oracle, "Sedov", **cfg)
+    st, od, cycles, final_dt, _ = po.run(oracle, p, init_fields=sedov_corner_fields(p))
+    assert st == 0 and cycles == 300
+    print(f"case Sedov corner 256^2 engine={engine}", flush=True)
+    q = armon.make_params("Sedov", engine=ENGINES[engine], **cfg)
+    assert q.bc_override == 1 and q.nb_cells == p.nb_cells
+    r, s = armon.run(q, sedov_corner_fields(q))
+    assert r.status == 0, armon.errors()
+    assert (r.cycles, r.final_dt) == (cycles, final_dt)
+    assert_same_state(s, od)
+    s.close()

@@ -353,3 +353,30 @@ def test_throughput_build_follows_the_host_cpu(po, monkeypatch):
                                                                        "liboracle_fast_v4.so"))
                   else "liboracle_fast.so")
     assert os.path.basename(po.lib_path(True)) == "liboracle.so"
+
+
+def test_boundary_factor_override(po, oracle):
+    """The boundary factors given in the parameters replace the test's. On the corner Sedov blast
+    (outside the reference's tests): with reflective walls the wall fluxes vanish exactly and the
+    mass stays constant to round-off; with the Sedov test's own free-flow walls the blast pushes
+    mass out through the corner walls; and overriding with the test's own factors changes
+    nothing."""
+    from conftest import SEDOV_CORNER, sedov_corner_fields
+
+    cfg = dict(scheme=po.SCHEME_GAD, projection=po.PROJ_EULER_2ND, limiter=po.LIM_MINMOD, nb_ghosts=4,
+               nx=48, ny=48, cfl=0.7, max_cycles=120, max_time=1.0, verbose=5)
+
+    def mass(p, fields=None, **kw):
+        q = po.make_params(oracle, "Sedov", **dict(cfg, **kw))
+        st, d, cycles, _, _ = po.run(oracle, q, init_fields=sedov_corner_fields(q) if fields is None else fields)
+        assert st == 0 and cycles == 120
+        m = d.array("domain_mask") == 1
+        return float(np.sum(d.array("rho")[m])), d.array("rho").copy()
+
+    walls = mass(None, **SEDOV_CORNER)
+    free = mass(None, domain_origin=SEDOV_CORNER["domain_origin"], domain_size=SEDOV_CORNER["domain_size"])
+    same = mass(None, domain_origin=SEDOV_CORNER["domain_origin"], domain_size=SEDOV_CORNER["domain_size"],
+                bc_factors=((1.0, 1.0),) * 4)
+    assert abs(walls[0] - 48 * 48) <= 1e-12 * 48 * 48
+    assert abs(free[0] - 48 * 48) > 1e-6 * 48 * 48
+    assert np.array_equal(same[1].view(np.uint64), free[1].view(np.uint64))
