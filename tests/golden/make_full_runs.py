@@ -4,7 +4,11 @@ for the oracle to run inside a GPU test, reduced to per-field digests of the fin
 Cases (GAD, Euler 2nd-order remap, Sequential splitting, 4 ghost layers):
 - Sedov 1024x1024, minmod, CFL 0.7, run to its end time t = 1 (thousands of cycles);
 - Bizarrium 4096x4096, superbee, CFL 0.6, 500 steps;
-- circular Sod 8192x8192, minmod, CFL 0.7, 200 steps (the FP64 side of the FP32-mixed run).
+- circular Sod 8192x8192, minmod, CFL 0.7, 200 steps (the FP64 side of the FP32-mixed run);
+- the corner Sedov blast (outside the reference's tests, through the parameter overrides and
+  initial fields of tests/conftest.py: blast in the 2x2 cells at the origin corner of [0, 1]^2,
+  reflective walls, background internal energy 1e-5) at 1024x1024, minmod, CFL 0.7, run to
+  t = 1.
 
 For each: the cycle count, the last dt (exact, as float.hex) and, per field, the SHA-256 of
 the real cells' FP64 values (row by row, little-endian) with their sum and sum of magnitudes
@@ -35,7 +39,22 @@ CASES = {
                                                                             nx=4096, ny=4096, max_cycles=500)),
     "sod_circ_8192_200_steps": dict(test="Sod_circ", params=dict(_COMMON, limiter=1, cfl=0.7, max_time=1e9,
                                                                  nx=8192, ny=8192, max_cycles=200)),
+    "sedov_corner_1024_to_end_time": dict(
+        test="Sedov", init="sedov_corner",
+        params=dict(_COMMON, limiter=1, cfl=0.7, nx=1024, ny=1024, max_cycles=10**6, max_time=1.0,
+                    domain_origin=[0.0, 0.0], domain_size=[1.0, 1.0],
+                    bc_factors=[[-1.0, 1.0], [-1.0, 1.0], [1.0, -1.0], [1.0, -1.0]])),
 }
+
+
+def initial_fields(case: dict, p):
+    """The fields a case writes after init_test (None: the test's own initial state)."""
+    if case.get("init") != "sedov_corner":
+        return None
+    sys.path.insert(0, os.path.join(HERE, ".."))
+    from conftest import sedov_corner_fields
+
+    return sedov_corner_fields(p)
 
 
 def real_cells(arr: np.ndarray, nx: int, ny: int, row_length: int, index_start: int) -> np.ndarray:
@@ -54,10 +73,11 @@ def run_case(po, lib, name: str) -> dict:
     case = CASES[name]
     p = po.make_params(lib, case["test"], **case["params"])
     t = time.time()
-    st, d, cycles, final_dt, _ = po.run(lib, p)
+    st, d, cycles, final_dt, _ = po.run(lib, p, init_fields=initial_fields(case, p))
     if st != 0:
         raise RuntimeError(f"{name}: oracle status {st}")
     out = {"test": case["test"], "params": case["params"], "cycles": cycles, "final_dt": float(final_dt).hex(),
+           "init": case.get("init"),
            "fields": {f: digest(real_cells(d.array(f), p.nx, p.ny, p.row_length, p.index_start)) for f in FIELDS}}
     print(f"{name}: {cycles} cycles in {time.time() - t:.0f} s", flush=True)
     return out

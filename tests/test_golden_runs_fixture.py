@@ -22,6 +22,7 @@ def test_fixture_holds_every_case():
     assert set(cases) == set(CASES)
     for name, c in cases.items():
         assert c["test"] == CASES[name]["test"] and c["params"] == CASES[name]["params"], name
+        assert c.get("init") == CASES[name].get("init"), name
         assert float.fromhex(c["final_dt"]) > 0
         assert set(c["fields"]) == set(FIELDS)
         for f, d in c["fields"].items():
@@ -106,3 +107,12 @@ def test_digest_is_over_the_exact_bits():
     b = np.array([[-0.0, 1.0]])
     assert digest(a)["sha256"] != digest(b)["sha256"]
     assert digest(a)["sum"] == digest(b)["sum"]
+
+
+def test_corner_sedov_keeps_its_mass():
+    """The corner Sedov blast is closed by reflective walls on all four sides, through which no
+    mass passes (the wall face velocity is exactly zero): at t = 1 the mass is still the initial
+    nx * ny * 1 to rounding."""
+    c = load_cases()["sedov_corner_1024_to_end_time"]
+    m0 = float(c["params"]["nx"] * c["params"]["ny"])
+    assert abs(c["fields"]["rho"]["sum"] - m0) <= 1e-12 * m0

@@ -5,7 +5,9 @@ states are kept as per-field SHA-256 digests of the real cells):
 - Sedov 1024x1024 to its end time t = 1, fused and staged engines;
 - Bizarrium 4096x4096, superbee, 500 steps;
 - circular Sod 8192x8192, 200 steps, and beside it the FP32-mixed form, whose error is taken
-  against that FP64 state (shown identical to the oracle's by its digests).
+  against that FP64 state (shown identical to the oracle's by its digests);
+- the corner Sedov blast with reflective walls (parameter overrides and initial fields),
+  1024x1024 to t = 1, fused and staged engines.
 
 Identical cycle count, last dt and digests mean every real cell of every field is bit-identical
 to the oracle's."""
@@ -18,7 +20,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from make_full_runs import FIELDS, digest, real_cells  # noqa: E402
+from make_full_runs import FIELDS, digest, initial_fields, real_cells  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +28,8 @@ with open(os.path.join(HERE, "golden", "full_runs.json")) as _f:
     GOLDEN = json.load(_f)["cases"]
 
 RUNS = [("sedov_1024_to_end_time", 1), ("sedov_1024_to_end_time", 0),
-        ("bizarrium_4096_superbee_500_steps", 1), ("sod_circ_8192_200_steps", 1)]
+        ("bizarrium_4096_superbee_500_steps", 1), ("sod_circ_8192_200_steps", 1),
+        ("sedov_corner_1024_to_end_time", 1), ("sedov_corner_1024_to_end_time", 0)]
 
 
 def state_cells(s, f):
@@ -38,7 +41,8 @@ def state_cells(s, f):
 def test_whole_run_matches_oracle_digests(armon, name, engine):
     g = GOLDEN[name]
     print(f"case {name} engine {engine}: {g['test']} {g['params']}", flush=True)
-    r, s = armon.run(armon.make_params(g["test"], engine=engine, **g["params"]))
+    q = armon.make_params(g["test"], engine=engine, **g["params"])
+    r, s = armon.run(q, initial_fields(g, q))
     try:
         assert r.status == 0, armon.errors()
         assert (r.cycles, float(r.final_dt).hex()) == (g["cycles"], g["final_dt"])

@@ -358,9 +358,8 @@ def test_throughput_build_follows_the_host_cpu(po, monkeypatch):
 def test_boundary_factor_override(po, oracle):
     """The boundary factors given in the parameters replace the test's. On the corner Sedov blast
     (outside the reference's tests): with reflective walls the wall fluxes vanish exactly and the
-    mass stays constant to round-off; with the Sedov test's own free-flow walls the blast pushes
-    mass out through the corner walls; and overriding with the test's own factors changes
-    nothing."""
+    mass stays constant to round-off; with the Sedov test's own free-flow walls mass crosses the
+    corner walls; and overriding with the test's own factors changes nothing."""
     from conftest import SEDOV_CORNER, sedov_corner_fields
 
     cfg = dict(scheme=po.SCHEME_GAD, projection=po.PROJ_EULER_2ND, limiter=po.LIM_MINMOD, nb_ghosts=4,
