@@ -14,7 +14,7 @@ For each: the cycle count, the last dt (exact, as float.hex) and, per field, the
 the real cells' FP64 values (row by row, little-endian) with their sum and sum of magnitudes
 for diagnostics. Run from the repository root after building the oracle:
 
-    python tests/golden/make_full_runs.py          # all cases, about 35 min on 8 cores
+    python tests/golden/make_full_runs.py          # all cases, about 65 min on 8 cores
     python tests/golden/make_full_runs.py sedov_1024_to_end_time
 
 The GPU test (tests/test_gpu_golden_runs.py) runs the same parameters through the library and

@@ -275,3 +275,46 @@ def test_sedov_corner_overrides(armon, po, oracle, engine):
     assert (r.cycles, r.final_dt) == (cycles, final_dt)
     assert_same_state(s, od)
     s.close()
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_sod_circ_radius_override(armon, po, oracle, engine):
+    """The circular Sod of radius 0.2 (the reference's is 0.3), through initial fields written
+    after init_test: 256x256, GAD + minmod, second-order remap, CFL 0.7, 200 cycles, bit for bit
+    against the oracle given the same fields."""
+    from conftest import sod_circ_radius_fields
+
+    cfg = dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=256, ny=256, cfl=0.7, max_cycles=200,
+               max_time=1.0, verbose=5)
+    p = po.make_params(oracle, "Sod_circ", **cfg)
+    st, od, cycles, final_dt, _ = po.run(oracle, p, init_fields=sod_circ_radius_fields(p, 0.2))
+    assert st == 0 and cycles == 200
+    print(f"case Sod_circ r=0.2 256^2 engine={engine}", flush=True)
+    q = armon.make_params("Sod_circ", engine=ENGINES[engine], **cfg)
+    r, s = armon.run(q, sod_circ_radius_fields(q, 0.2))
+    assert r.status == 0, armon.errors()
+    assert (r.cycles, r.final_dt) == (cycles, final_dt)
+    assert_same_state(s, od)
+    s.close()
+
+
+@pytest.mark.parametrize("engine", ["fused", "staged"])
+def test_bizarrium_left_moving_override(armon, po, oracle, engine):
+    """The Bizarrium tube with its left state moving at u = 250 and the right one at rest (the
+    reference moves the right one), through initial fields written after init_test: 256x256,
+    GAD + superbee, second-order remap, CFL 0.6, 200 cycles, bit for bit against the oracle given
+    the same fields."""
+    from conftest import bizarrium_fields
+
+    cfg = dict(scheme=1, projection=2, limiter=2, nb_ghosts=4, nx=256, ny=256, cfl=0.6, max_cycles=200,
+               max_time=1.0, verbose=5)
+    p = po.make_params(oracle, "Bizarrium", **cfg)
+    st, od, cycles, final_dt, _ = po.run(oracle, p, init_fields=bizarrium_fields(p, left_moving=True))
+    assert st == 0 and cycles == 200
+    print(f"case Bizarrium left-moving 256^2 engine={engine}", flush=True)
+    q = armon.make_params("Bizarrium", engine=ENGINES[engine], **cfg)
+    r, s = armon.run(q, bizarrium_fields(q, left_moving=True))
+    assert r.status == 0, armon.errors()
+    assert (r.cycles, r.final_dt) == (cycles, final_dt)
+    assert_same_state(s, od)
+    s.close()

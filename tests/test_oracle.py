@@ -379,3 +379,29 @@ def test_boundary_factor_override(po, oracle):
     assert abs(walls[0] - 48 * 48) <= 1e-12 * 48 * 48
     assert abs(free[0] - 48 * 48) > 1e-6 * 48 * 48
     assert np.array_equal(same[1].view(np.uint64), free[1].view(np.uint64))
+
+
+def test_sod_circ_radius_fields_reproduce_init_test(po, oracle):
+    """The initial fields the circular Sod radius override writes are those of init_test at the
+    reference radius 0.3, ghost cells included."""
+    from conftest import sod_circ_radius_fields
+
+    p = po.make_params(oracle, "Sod_circ", nx=96, ny=80, nb_ghosts=4, verbose=5)
+    d = po.Data(oracle, p.nb_cells)
+    oracle.o_init_test_params(C.byref(p), C.byref(d.c), False)
+    f = sod_circ_radius_fields(p, 0.3)
+    for name in ("rho", "umat", "vmat", "Emat"):
+        assert np.array_equal(f[name], d.array(name)), name
+
+
+def test_bizarrium_fields_reproduce_init_test(po, oracle):
+    """The Bizarrium initial fields of the override helper, in the reference's orientation, are
+    those of init_test, ghost cells included."""
+    from conftest import bizarrium_fields
+
+    p = po.make_params(oracle, "Bizarrium", nx=90, ny=70, nb_ghosts=4, verbose=5)
+    d = po.Data(oracle, p.nb_cells)
+    oracle.o_init_test_params(C.byref(p), C.byref(d.c), False)
+    f = bizarrium_fields(p, left_moving=False)
+    for name in ("rho", "umat", "vmat", "Emat"):
+        assert np.array_equal(f[name], d.array(name)), name
