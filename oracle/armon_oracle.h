@@ -22,7 +22,10 @@
  *   - conservation_vars sums in a fixed, thread-count independent order with
  *     Neumaier compensation (Kokkos' sum order is backend-defined).
  *
- * Parity status of every function is listed in the test suite and DESIGN.md.
+ * Every function is pinned by the tests (tests/test_oracle.py: the reference's known
+ * answers, closed forms, exact rational evaluations, invariants, brute force), except two
+ * edge semantics taken from Kokkos' definitions rather than from a test of the reference:
+ * the DBL_MAX identity of its Min reducer and the min/max forms above.
  */
 #ifndef ARMON_ORACLE_H
 #define ARMON_ORACLE_H
