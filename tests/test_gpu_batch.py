@@ -18,11 +18,14 @@ def small_specs(armon, seed, n, min_side=24, max_side=96):
 
 def oracle_domain(po, oracle, bp, spec, **cfg):
     """The oracle run of one domain with the batch's parameters for it."""
+    extra = {}
+    if bp.bc_override:
+        extra["bc_factors"] = [(bp.bc_factors[side][0], bp.bc_factors[side][1]) for side in range(4)]
     p = po.make_params(oracle, int(bp.test), nx=bp.nx, ny=bp.ny, nb_ghosts=bp.nb_ghosts, scheme=bp.scheme,
                        projection=bp.projection, limiter=bp.limiter, axis_splitting=bp.axis_splitting,
                        stencil_width=bp.stencil_width, cfl=bp.cfl, max_time=bp.max_time,
                        max_cycles=bp.max_cycles, cst_dt=bp.cst_dt, Dt=bp.Dt,
-                       dt_on_even_cycles=bp.dt_on_even_cycles, verbose=5)
+                       dt_on_even_cycles=bp.dt_on_even_cycles, verbose=5, **extra)
     st = po.OTestState(**{f: getattr(spec.state, f) for f, _ in po.OTestState._fields_})
     status, d, cycles, final_dt, _ = po.run(oracle, p, states=st)
     return status, d, cycles, final_dt
@@ -48,8 +51,12 @@ def compare_batch(armon, po, oracle, b, specs, fields=STATE):
     assert not msgs, "; ".join(msgs[:8])
 
 
-def run_batch(armon, specs, tuning=None, autotune=False, advance=None, **cfg):
+def run_batch(armon, specs, tuning=None, autotune=False, advance=None, bc_factors=None, **cfg):
     common = armon.batch_params(**cfg)
+    if bc_factors is not None:
+        common.bc_override = 1
+        for side, (uf, vf) in enumerate(bc_factors):
+            common.bc_factors[side][0], common.bc_factors[side][1] = uf, vf
     b = armon.Batch(common, specs)
     b.init()
     if autotune:
@@ -183,3 +190,18 @@ def test_batch_fp32_mixed_independent_of_launch_configuration(armon):
         bad = [i for i, (a, o) in enumerate(zip(out, ref)) if not np.array_equal(a.view(np.uint64), o.view(np.uint64))]
         assert not bad, (tuning, [(i // len(STATE), STATE[i % len(STATE)]) for i in bad[:8]])
 
+
+
+def test_batch_boundary_factor_override(armon, po, oracle):
+    """Boundary factors given in the batch's common parameters replace every domain's test
+    factors (here reflective walls on all four sides, which no reference test has): each domain
+    ends bit for bit where the oracle ends with the same override."""
+    specs = small_specs(armon, 31, 6)
+    print("case batch seed 31 n 6 reflective walls", flush=True)
+    walls = ((-1.0, 1.0), (-1.0, 1.0), (1.0, -1.0), (1.0, -1.0))
+    b = run_batch(armon, specs, bc_factors=walls, **BASE)
+    for k in range(b.n):
+        bp = b.params(k)
+        assert bp.bc_override == 1 and [tuple(bp.bc_factors[s]) for s in range(4)] == list(walls)
+    compare_batch(armon, po, oracle, b, specs)
+    b.close()
