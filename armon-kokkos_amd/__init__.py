@@ -159,6 +159,7 @@ EXTENSION_SYMBOLS = {
                                 _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, C.c_int, _d,
                                 C.POINTER(ArmonTestState)]),
     "dt_CFL_device": (None, [_R, _I1, _d, _d, _V, _V, _V, _V, C.c_void_p]),
+    "conservation_vars_noalloc": (None, [_R, _I1, _d, _V, _V, _V, C.POINTER(_d), C.POINTER(_d)]),
     "armon_params_default": (None, [_PP]),
     "armon_params_init": (C.c_int, [_PP]),
     "armon_params_check": (C.c_int, [_PP]),

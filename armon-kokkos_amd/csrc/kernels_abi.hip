@@ -608,6 +608,36 @@ __global__ void k_conservation_final(int nblocks, const double* partials, double
     out[1] = e.s + e.c;
 }
 
+// ---- allocation-free conservation sums ----------------------------------------------------
+
+// Scratch of conservation_vars_noalloc: 4 doubles of partials per block (at most kRedBlocks
+// blocks) and the 2 results, 32,784 bytes, held by every device the library runs on until the
+// process ends. Host code never uses this name as an address: the device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch below).
+constexpr int kNoAllocBlocks = kRedBlocks;
+constexpr int kNoAllocDoubles = 4 * kNoAllocBlocks + 2;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+
+// The device address of `scratch` on the current device, looked up on the first call on that
+// device (with it current) and kept.
+double* noalloc_scratch()
+{
+    static std::mutex mtx;
+    static double* ptr[kMaxDevices] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw invalid_argument("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(mtx);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
 const char* limiter_error(int limiter)
 {
     static thread_local char buf[160];
@@ -1016,7 +1046,7 @@ static void launch_dt_CFL(const Range& range, const InnerRange1D& inner_range, f
     check_view(vmat, "vmat", sp);
     check_view(cmat, "cmat", sp);
     check_view(domain_mask, "domain_mask", sp);
-    Scratch& sc = scratch();
+    Scratch& sc = armon::scratch();
     const double init = (sizeof(flt_t) == 8) ? DBL_MAX : (double) FLT_MAX;
     k_fill_bits<<<1, 1, 0, stream()>>>(sc.min_bits, (unsigned long long) __builtin_bit_cast(long long, init));
     after_launch("dt_CFL(init)");
@@ -1036,7 +1066,7 @@ ARMON_EXPORT flt_t dt_CFL(const Range& range, const InnerRange1D& inner_range, f
                           const view& umat, const view& vmat, const view& cmat, const mask_view& domain_mask)
 {
     try {
-        Scratch& sc = scratch();
+        Scratch& sc = armon::scratch();
         launch_dt_CFL(range, inner_range, dx, dy, umat, vmat, cmat, domain_mask, sc.result);
         double out = 0;
         hip_check(hipMemcpyAsync(&out, sc.result, sizeof(double), hipMemcpyDeviceToHost, stream()), "dt_CFL copy");
@@ -1058,34 +1088,62 @@ ARMON_KERNEL_TRY
 }
 ARMON_KERNEL_CATCH("dt_CFL_device")
 
-ARMON_EXPORT void conservation_vars(const Range& range, const InnerRange1D& inner_range, flt_t dx, const view& rho,
-                                    const view& Emat, const mask_view& domain_mask, flt_t& total_mass,
-                                    flt_t& total_energy)
-ARMON_KERNEL_TRY
+// The compensated sums over `range` with `max_blocks` blocks of partials at `partials` and the
+// two results at `result` (device memory), returned to the host.
+static void run_conservation(const char* name, const Range& range, const InnerRange1D& inner_range, flt_t dx,
+                             const view& rho, const view& Emat, const mask_view& domain_mask, double* partials,
+                             int max_blocks, double* result, flt_t& total_mass, flt_t& total_energy)
 {
     const Iter1D it = make_iter1d(range, inner_range);
     const Span sp = span_of(it);
     check_view(rho, "rho", sp);
     check_view(Emat, "Emat", sp);
     check_view(domain_mask, "domain_mask", sp);
-    Scratch& sc = scratch();
     int64_t n = it.end - it.start;
     int64_t g = (n + kBlock - 1) / kBlock;
     if (g > kRedBlocks) g = kRedBlocks;
     if (g < 1) g = 1;
+    if (g > max_blocks)
+        throw out_of_range(std::string(name) + ": " + std::to_string(g) + " reduction blocks exceed the scratch's "
+                           + std::to_string(max_blocks));
     const double ds = (double) dx * (double) dx;
     k_conservation_partial<<<(unsigned) g, kBlock, 0, stream()>>>(it, ds, rho.data, Emat.data, domain_mask.data,
-                                                                   sc.partials);
-    after_launch("conservation_vars");
-    k_conservation_final<<<1, 1, 0, stream()>>>((int) g, sc.partials, sc.result);
-    after_launch("conservation_vars(final)");
+                                                                   partials);
+    after_launch(name);
+    k_conservation_final<<<1, 1, 0, stream()>>>((int) g, partials, result);
+    after_launch(name);
     double out[2];
-    hip_check(hipMemcpyAsync(out, sc.result, sizeof out, hipMemcpyDeviceToHost, stream()), "conservation copy");
+    hip_check(hipMemcpyAsync(out, result, sizeof out, hipMemcpyDeviceToHost, stream()), "conservation copy");
     hip_check(hipStreamSynchronize(stream()), "conservation sync");
     total_mass = (flt_t) out[0];
     total_energy = (flt_t) out[1];
 }
+
+ARMON_EXPORT void conservation_vars(const Range& range, const InnerRange1D& inner_range, flt_t dx, const view& rho,
+                                    const view& Emat, const mask_view& domain_mask, flt_t& total_mass,
+                                    flt_t& total_energy)
+ARMON_KERNEL_TRY
+{
+    Scratch& sc = armon::scratch();
+    run_conservation("conservation_vars", range, inner_range, dx, rho, Emat, domain_mask, sc.partials,
+                     Scratch::kPartials / 4, sc.result, total_mass, total_energy);
+}
 ARMON_KERNEL_CATCH("conservation_vars")
+
+// conservation_vars with its partials and results in the library's static device scratch
+// instead of memory allocated on first use: no allocation at any call. The partials are
+// written before they are read in every call, so nothing needs clearing; calls that overlap
+// on one device (other host threads or streams) share the scratch and race.
+ARMON_EXPORT void conservation_vars_noalloc(const Range& range, const InnerRange1D& inner_range, flt_t dx,
+                                            const view& rho, const view& Emat, const mask_view& domain_mask,
+                                            flt_t& total_mass, flt_t& total_energy)
+ARMON_KERNEL_TRY
+{
+    double* base = noalloc_scratch();
+    run_conservation("conservation_vars_noalloc", range, inner_range, dx, rho, Emat, domain_mask, base,
+                     kNoAllocBlocks, base + 4 * kNoAllocBlocks, total_mass, total_energy);
+}
+ARMON_KERNEL_CATCH("conservation_vars_noalloc")
 
 // ---- measurement -----------------------------------------------------------------------
 

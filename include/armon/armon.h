@@ -57,6 +57,19 @@ ARMON_EXPORT void dt_CFL_device(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D
                                 ARMON_CREF(view) umat, ARMON_CREF(view) vmat, ARMON_CREF(view) cmat,
                                 ARMON_CREF(mask_view) domain_mask, double* dt_device);
 
+/* ---- allocation-free conservation sums ------------------------------------ */
+
+/* conservation_vars (same arguments, results and error behaviour) that allocates nothing at
+ * any call, for callers that cannot allocate device memory (tight loops, harnesses that forbid
+ * hipMalloc). It has no counterpart in the reference. Its partial sums live in a static device
+ * array of the library, 4 x 1024 + 2 doubles (32,784 bytes) per device, held until the process
+ * ends; any range fits, since the sum folds any number of cells into at most 1024 block
+ * partials. Calls that overlap on one device share that array and race. */
+ARMON_EXPORT void conservation_vars_noalloc(ARMON_CREF(Range) range, ARMON_CREF(InnerRange1D) inner_range, flt_t dx,
+                                            ARMON_CREF(view) rho, ARMON_CREF(view) Emat,
+                                            ARMON_CREF(mask_view) domain_mask,
+                                            ARMON_REF(flt_t) total_mass, ARMON_REF(flt_t) total_energy);
+
 /* ---- initial states -------------------------------------------------------- */
 
 /* High/low region states of a test case (init_test.cpp:12-48, TestParams of test_cases.h:17-22). */
