@@ -405,3 +405,192 @@ def test_bizarrium_fields_reproduce_init_test(po, oracle):
     f = bizarrium_fields(p, left_moving=False)
     for name in ("rho", "umat", "vmat", "Emat"):
         assert np.array_equal(f[name], d.array(name)), name
+
+
+# ---- border arrays (halo_exchange.cpp:39-92) ---------------------------------------------
+
+BORDER_FIELDS = ("rho", "umat", "vmat", "pmat", "cmat", "gmat", "Emat")   # value_array order, :56-62
+
+
+def _distinct_fields(po, oracle, p, offset=0.0):
+    """Arrays of `p` whose every value names its field and cell: field f at index i holds
+    (f + 1) * 1e7 + i + offset."""
+    d = po.Data(oracle, p.nb_cells)
+    for f, name in enumerate(BORDER_FIELDS):
+        d.array(name)[:] = (f + 1) * 1e7 + np.arange(p.nb_cells) + offset
+    return d
+
+
+def _band_range(po, p, first_col):
+    """The band of nghost columns starting at column `first_col` over the ny real rows, iterated
+    as the border routines expect: inner index = column within the band (row_range_length =
+    nghost), outer index = position along the side (one row per step)."""
+    g = p.nb_ghosts
+    r = po.ORange(0, p.ny * g)
+    ir = po.OInnerRange2D(p.index_start + first_col, p.row_length, 0, g)
+    band = p.index_start + np.arange(p.ny)[:, None] * p.row_length + first_col + np.arange(g)[None, :]
+    return r, ir, band   # band[position, layer] -> array index
+
+
+def _border(oracle, fn, p, r, ir, d, values):
+    getattr(oracle, fn)(C.byref(r), C.byref(ir), p.ny, p.nb_ghosts,
+                        *[getattr(d.c, n) for n in BORDER_FIELDS], values.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+def test_read_border_array_layout(po, oracle):
+    """read_border_array fills every slot of the 7 x nghost x side_length value array exactly once,
+    as layer-major blocks: value[layer][position][field], the fields in the reference's order."""
+    p = po.make_params(oracle, "Sod", nx=20, ny=14, nb_ghosts=4)
+    d = _distinct_fields(po, oracle, p)
+    r, ir, band = _band_range(po, p, p.nx - p.nb_ghosts)
+    values = np.full(7 * p.nb_ghosts * p.ny, np.nan)
+    _border(oracle, "o_read_border_array", p, r, ir, d, values)
+    assert not np.isnan(values).any()
+    expect = np.stack([d.array(n)[band] for n in BORDER_FIELDS])        # (field, position, layer)
+    assert np.array_equal(values.reshape(p.nb_ghosts, p.ny, 7), expect.transpose(2, 1, 0))
+    assert len(set(values.tolist())) == values.size                       # every cell and field once
+
+
+def test_border_array_round_trip_and_exchange(po, oracle):
+    """write(read(x)) restores the band and touches nothing else; and the halo exchange the two
+    routines exist for: the real band read on one domain's right side and written into another
+    domain's left ghost band puts real column nx - nghost + l of the first into ghost column
+    -nghost + l of the second."""
+    p = po.make_params(oracle, "Sod", nx=20, ny=14, nb_ghosts=4)
+    g = p.nb_ghosts
+    a = _distinct_fields(po, oracle, p)
+    before = {n: a.array(n).copy() for n in BORDER_FIELDS}
+    r, ir, band = _band_range(po, p, p.nx - g)
+    values = np.zeros(7 * g * p.ny)
+    _border(oracle, "o_read_border_array", p, r, ir, a, values)
+    for n in BORDER_FIELDS:
+        a.array(n)[band] = -1.0
+    _border(oracle, "o_write_border_array", p, r, ir, a, values)
+    for n in BORDER_FIELDS:
+        assert np.array_equal(a.array(n), before[n]), n
+    b = _distinct_fields(po, oracle, p, offset=0.5)
+    b_before = {n: b.array(n).copy() for n in BORDER_FIELDS}
+    rg, irg, ghost_band = _band_range(po, p, -g)
+    _border(oracle, "o_write_border_array", p, rg, irg, b, values)
+    for n in BORDER_FIELDS:
+        got = b.array(n)
+        assert np.array_equal(got[ghost_band], before[n][band]), n
+        untouched = np.ones(p.nb_cells, bool)
+        untouched[ghost_band.ravel()] = False
+        assert np.array_equal(got[untouched], b_before[n][untouched]), n
+
+
+# ---- axis splitting (parameters.cpp:212-255) -----------------------------------------------
+
+def test_split_axes_properties(po, oracle):
+    """Per cycle, each axis's time factors sum to one (or zero for the axis a one-axis splitting
+    leaves out); Sequential is always X then Y; SequentialSym and Strang start with Y on even
+    cycles and with X on odd ones; Strang is a palindrome of half steps around a full one."""
+    for split in range(5):
+        for cycle in range(6):
+            p = po.make_params(oracle, "Sod", axis_splitting=split)
+            p.cycle = cycle
+            axes, f = (C.c_int * 3)(), (C.c_double * 3)()
+            n = oracle.o_split_axes(C.byref(p), axes, f)
+            seq = [(axes[k], f[k]) for k in range(n)]
+            sx = sum(w for a, w in seq if a == po.AXIS_X)
+            sy = sum(w for a, w in seq if a == po.AXIS_Y)
+            assert (sx, sy) == ((1.0, 0.0) if split == po.SPLIT_X_ONLY else
+                                (0.0, 1.0) if split == po.SPLIT_Y_ONLY else (1.0, 1.0)), (split, cycle, seq)
+            first = po.AXIS_Y if cycle % 2 == 0 else po.AXIS_X
+            if split == po.SPLIT_SEQUENTIAL:
+                assert [a for a, _ in seq] == [po.AXIS_X, po.AXIS_Y]
+            elif split in (po.SPLIT_SEQUENTIAL_SYM, po.SPLIT_STRANG):
+                assert seq[0][0] == first and seq[1][0] != first
+            if split == po.SPLIT_STRANG:
+                assert seq[0] == seq[2] == (first, 0.5) and seq[1][1] == 1.0
+
+
+@pytest.mark.parametrize("scheme,proj", [(1, 2), (0, 1)])
+def test_one_axis_splittings_transpose(po, oracle, scheme, proj):
+    """X_only on Sod (along x, reflective left/right) and Y_only on Sod_y (along y, reflective
+    top/bottom) are exact transposes on a square domain: same cycle count, same final dt, every
+    field bit for bit transposed (u and v exchanged). With one sweep per cycle the lagged CFL reads
+    the sound speed of the same sweep in both, unlike the two-axis splittings."""
+    cfg = dict(scheme=scheme, projection=proj, limiter=1, nb_ghosts=5, nx=48, ny=48, max_cycles=400, verbose=5)
+    px = po.make_params(oracle, "Sod", axis_splitting=po.SPLIT_X_ONLY, **cfg)
+    py = po.make_params(oracle, "Sod_y", axis_splitting=po.SPLIT_Y_ONLY, **cfg)
+    sx, dx_, cx, fx, _ = po.run(oracle, px)
+    sy, dy_, cy, fy, _ = po.run(oracle, py)
+    assert sx == sy == 0 and cx == cy and cx > 20 and fx == fy
+    g = px.nb_ghosts
+    for a, b in (("rho", "rho"), ("Emat", "Emat"), ("pmat", "pmat"), ("umat", "vmat"), ("vmat", "umat")):
+        fa = _field(dx_, a, px)[g:g + px.ny, g:g + px.nx]
+        fb = _field(dy_, b, py)[g:g + py.ny, g:g + py.nx]
+        assert np.array_equal(fa, fb.T), (a, b)
+
+
+# ---- time step rules (armon_2D.cpp:212-252) -----------------------------------------------
+
+def _brute_cfl(d, p):
+    """min over the real cells of min(dx / max|u +- c|, dy / max|v +- c|) (reductions.cpp:18-20)."""
+    m = d.array("domain_mask") == 1
+    u, v, c = d.array("umat")[m], d.array("vmat")[m], d.array("cmat")[m]
+    dx, dy = p.domain_size[0] / p.nx, p.domain_size[1] / p.ny
+    return min(np.min(dx / np.maximum(np.abs(u + c), np.abs(u - c))),
+               np.min(dy / np.maximum(np.abs(v + c), np.abs(v - c))))
+
+
+def _stepped_run(po, oracle, p):
+    """The time loop of armon_2D.cpp:327-386 cycle by cycle: the dt each cycle used and the brute
+    force CFL bound of the state each cycle started from. Ends where o_time_loop ends."""
+    d = po.Data(oracle, p.nb_cells)
+    oracle.o_init_test_params(C.byref(p), C.byref(d.c), False)
+    p.cycle, p.time, p.current_cycle_dt, p.next_cycle_dt = 0, 0.0, 0.0, 0.0
+    oracle.o_update_axis(C.byref(p), po.AXIS_X)
+    oracle.o_update_EOS(C.byref(p), C.byref(d.c))
+    dts, bounds = [], []
+    while p.time < p.max_time and p.cycle < p.max_cycles:
+        bounds.append(_brute_cfl(d, p))
+        assert oracle.o_solver_cycle(C.byref(p), C.byref(d.c)) == 0
+        dts.append(p.current_cycle_dt)
+        p.time += p.current_cycle_dt
+        p.cycle += 1
+    return dts, bounds, d
+
+
+CYCLE_CFG = dict(scheme=1, projection=2, limiter=1, nb_ghosts=4, nx=40, ny=36, max_cycles=40, verbose=5)
+
+
+def test_lagged_cfl_time_step(po, oracle):
+    """Every cycle: dt(0) = dt(1) = cfl * CFL(state 0), then dt(c + 1) = min(cfl * CFL(state c),
+    1.05 dt(c)), the bound taken from the state the cycle starts from; the same run as o_time_loop."""
+    p = po.make_params(oracle, "Sedov", **CYCLE_CFG)
+    dts, bounds, d = _stepped_run(po, oracle, p)
+    assert dts[0] == dts[1] == p.cfl * bounds[0]
+    for c in range(1, len(dts) - 1):
+        assert dts[c + 1] == min(p.cfl * bounds[c], 1.05 * dts[c]), c
+    q = po.make_params(oracle, "Sedov", **CYCLE_CFG)
+    st, od, cycles, final_dt, _ = po.run(oracle, q)
+    assert st == 0 and cycles == len(dts) and np.array_equal(od.array("rho"), d.array("rho"))
+
+
+def test_dt_on_even_cycles(po, oracle):
+    """dt_on_even_cycles: the bound is taken on even cycles only and the dt changes only between
+    an even cycle and the next: dt(2k + 2) = dt(2k + 1), dt(2k + 1) = min(cfl * CFL(state 2k),
+    1.05 dt(2k)) for k >= 1, and dt(0) = dt(1) = dt(2) = cfl * CFL(state 0)."""
+    p = po.make_params(oracle, "Sedov", dt_on_even_cycles=1, **CYCLE_CFG)
+    dts, bounds, _ = _stepped_run(po, oracle, p)
+    assert len(dts) == CYCLE_CFG["max_cycles"]
+    assert dts[0] == dts[1] == dts[2] == p.cfl * bounds[0]
+    for k in range(1, (len(dts) - 1) // 2):
+        assert dts[2 * k + 1] == min(p.cfl * bounds[2 * k], 1.05 * dts[2 * k]), k
+        if 2 * k + 2 < len(dts):
+            assert dts[2 * k + 2] == dts[2 * k + 1], k
+    assert len(set(dts)) > 5   # the bound does move
+
+
+def test_constant_time_step(po, oracle):
+    """cst_dt: every cycle uses Dt, so a run to t_final takes ceil(t_final / Dt) cycles (Dt a power
+    of two: the time sums exactly)."""
+    Dt = 2.0 ** -10
+    p = po.make_params(oracle, "Sod", cst_dt=1, Dt=Dt, scheme=1, projection=2, nb_ghosts=4, nx=32, ny=24,
+                       max_cycles=10000, verbose=5)
+    dts, _, _ = _stepped_run(po, oracle, p)
+    assert all(dt == Dt for dt in dts)
+    assert len(dts) == int(np.ceil(p.max_time / Dt)) == 205
