@@ -40,7 +40,7 @@ Options:
     --verbose 0-3           Verbosity (0: high, 3: low)
     --engine <engine>       'fused' (default: one kernel per axis sweep) or 'staged' (one launch per stage)
     --arith <mode>          'fp64' (default) or 'fp32mixed' (fused engine only)
-    --graph-chunk N         Cycles per captured hipGraph launch (fused engine, even; 0 disables)
+    --graph-chunk N         Cycles per captured hipGraph launch (fused engine; 0 disables)
     --y-run N               Rows per marching run of the fused Y sweep (0: chosen from the domain size)
     --batch N               Run a batched ensemble of N domains (64^2 to 1024^2 cells, Sod / Sod_circ /
                             Sedov, perturbed states) drawn from --seed S (default 2024), each to its own
@@ -219,8 +219,9 @@ static bool parse_arguments(ArmonParams& p, BatchOptions& bo, int argc, char** a
         fprintf(stderr, "Parsing error occurred: %s\n", strerror(errno));
         return false;
     }
-    // the per-step comparison checks every stage: it runs the staged engine unless told otherwise
-    if (p.compare && !engine_given) p.engine = ARMON_ENGINE_STAGED;
+    // the per-step comparison checks every stage, and the fused engine keeps no ghost cells in
+    // memory: both run the staged engine unless told otherwise
+    if ((p.compare || (p.write_output && p.write_ghosts)) && !engine_given) p.engine = ARMON_ENGINE_STAGED;
     return true;
 }
 
@@ -241,8 +242,8 @@ static int run_batch(ArmonParams common, const BatchOptions& bo)
         ArmonBatchTuning t;
         if (armon_batch_autotune(b, &t) != 0) goto done;
         if (common.verbose < 3)
-            printf("Autotuned: Y runs of %d rows, X grid %d, Y grid %d (%.3f + %.3f ms per sweep pair)\n",
-                   t.y_run_rows, t.grid_x, t.grid_y, t.ms_x, t.ms_y);
+            printf("Autotuned: X tiles of %d cells, Y runs of %d rows, X grid %d, Y grid %d (%.3f + %.3f ms per "
+                   "sweep pair)\n", t.tile_x, t.y_run_rows, t.grid_x, t.grid_y, t.ms_x, t.ms_y);
     }
     {
         if (armon_batch_begin(b) != 0) goto done;

@@ -43,12 +43,14 @@ extern "C" {
 #define ARMON_REF(T) T*
 #endif
 
-/* common.h:8-12 */
+/* common.h:8-12. The library is built in FP64 only: the reference's single-precision build
+ * (USE_SINGLE_PRECISION) has no counterpart here, so a caller asking for it would bind float
+ * declarations to double implementations. The reduced-precision form of this library is the
+ * fused engine's FP32-mixed precision (armon.h). */
 #if defined(ARMON_SINGLE_PRECISION) && ARMON_SINGLE_PRECISION
-typedef float flt_t;
-#else
-typedef double flt_t;
+#error "libarmon is built in FP64 only: ARMON_SINGLE_PRECISION (the reference's USE_SINGLE_PRECISION) is not supported"
 #endif
+typedef double flt_t;
 
 /* indexing.h:14-19 */
 typedef int64_t Idx;

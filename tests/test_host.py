@@ -243,3 +243,32 @@ def test_cli_help():
     assert out.returncode == 0 and "--engine" in out.stdout and "--cells" in out.stdout
     out = subprocess.run([exe, "--bogus"], capture_output=True, text=True, timeout=60)
     assert out.returncode != 0 and "Wrong option" in out.stderr
+
+
+def test_fused_engine_refuses_ghost_output(armon):
+    """The fused engine keeps no ghost cells in memory, so writing them (write_ghosts with
+    write_output) is refused for it and accepted for the staged engine."""
+    lib = armon.load()
+    for engine, ok in ((armon.ENGINE_FUSED, 0), (armon.ENGINE_STAGED, 1)):
+        p = armon.make_params("Sod", nx=32, ny=32, nb_ghosts=4, engine=engine, write_output=1, write_ghosts=1)
+        assert lib.armon_params_check(C.byref(p)) == ok
+        p.write_ghosts = 0
+        assert lib.armon_params_check(C.byref(p)) == 1
+
+
+def test_single_precision_switch_is_refused(tmp_path):
+    """The library is FP64 only: including kernels.h with ARMON_SINGLE_PRECISION set (the
+    reference's USE_SINGLE_PRECISION, common.h:8-12) is a compile error instead of float
+    declarations bound to double implementations; without it flt_t is double."""
+    import subprocess
+
+    src = tmp_path / "use.c"
+    src.write_text('#include "armon/kernels.h"\n'
+                   '_Static_assert(sizeof(flt_t) == 8, "flt_t is double");\nint main(void) { return 0; }\n')
+    inc = os.path.join(ROOT, "include")
+    ok = subprocess.run(["gcc", "-std=c11", "-I", inc, "-c", str(src), "-o", str(tmp_path / "a.o")],
+                        capture_output=True, text=True)
+    assert ok.returncode == 0, ok.stderr
+    bad = subprocess.run(["gcc", "-std=c11", "-DARMON_SINGLE_PRECISION=1", "-I", inc, "-c", str(src), "-o",
+                          str(tmp_path / "b.o")], capture_output=True, text=True)
+    assert bad.returncode != 0 and "FP64 only" in bad.stderr
