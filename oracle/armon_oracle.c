@@ -38,6 +38,27 @@
         _Pragma("omp simd")                                                                             \
         for (o_idx i = base_ + row_ * (r).main_range_step + (row_ == row0_ ? k0_ - row0_ * len_ : 0);    \
              i < base_ + row_ * (r).main_range_step + (row_ == row1_ ? k1_ - row1_ * len_ : len_); i++)
+#elif ORACLE_MD_ITER
+/* The throughput build's other iteration, as the reference's USE_MD_ITER + BALANCE_MD_ITER build
+ * walks it (iter_md, indexing.h:157-189): the bounding rectangle of the range in (row, column)
+ * coordinates of the main step, its rows grouped in tiles of o_md_tile_rows() rows (the
+ * balanced split of indexing.cpp:113-191) shared statically among the threads. The column tile,
+ * left to Kokkos' default by the reference, is the whole row here, a vector loop. */
+static o_idx o_md_tile_rows(o_idx rows);
+#define FOR_2D(range, r, i)                                                                             \
+    const int nonempty_ = (range)->end > (range)->start;                                                \
+    const o_idx first_ = nonempty_ ? o_scale_index_2d(&(r), (o_uidx) (range)->start) : 0;               \
+    const o_idx last_ = nonempty_ ? o_scale_index_2d(&(r), (o_uidx) ((range)->end - 1)) : 0;            \
+    const o_idx step_ = (r).main_range_step;                                                            \
+    const o_idx y0_ = first_ / step_, x0_ = first_ % step_;                                             \
+    const o_idx y1_ = last_ / step_ + 1, x1_ = last_ % step_ + 1;                                       \
+    const o_idx ty_ = nonempty_ ? o_md_tile_rows(y1_ - y0_) : 1;                                        \
+    const o_idx nt_ = nonempty_ ? (y1_ - y0_ + ty_ - 1) / ty_ : 0;                                      \
+    PARALLEL_FOR                                                                                        \
+    for (o_idx t_ = 0; t_ < nt_; t_++)                                                                  \
+        for (o_idx y_ = y0_ + t_ * ty_; y_ < y0_ + (t_ + 1) * ty_ && y_ < y1_; y_++)                    \
+            _Pragma("omp simd")                                                                         \
+            for (o_idx i = y_ * step_ + x0_; i < y_ * step_ + x1_; i++)
 #else
 #define FOR_2D(range, r, i)                                                                             \
     PARALLEL_FOR                                                                                        \
@@ -60,6 +81,47 @@ static inline int ieee_isfinite(double x)
     uint64_t b; memcpy(&b, &x, sizeof b);
     return (b & 0x7FF0000000000000ull) != 0x7FF0000000000000ull;
 }
+
+#if ORACLE_MD_ITER
+/* indexing.cpp:113-130: the share of the last round of tiles that leaves threads idle */
+static double o_overwork_factor(o_idx n, o_idx threads, o_idx tile)
+{
+    const double useful = (double) n / (double) tile;
+    const double total = ceil(useful / (double) threads) * (double) threads;
+    return 1 - useful / total;
+}
+
+/* indexing.cpp:137-191 (optimal_split_iter_space, tolerance 0.10): the largest tile of rows, from
+ * `threads` rows down, whose overwork is zero, or else below the tolerance (a smaller one only if
+ * it halves the overwork), or else the one of least overwork. */
+static o_idx o_md_tile_rows(o_idx rows)
+{
+#ifdef _OPENMP
+    const o_idx threads = omp_get_max_threads();
+#else
+    const o_idx threads = 1;
+#endif
+    const o_idx min_tile = threads < 4 ? threads : 4;
+    double min_of = INFINITY, best_of = 0;
+    o_idx optimal = 0, best = 0;
+    for (o_idx tile = threads; tile > 0; tile--) {
+        if (tile < min_tile) continue;
+        const double of = o_overwork_factor(rows, threads, tile);
+        if (of == 0) {
+            best = tile;
+            break;
+        } else if (of < 0.10 && (best < tile || 2 * of < best_of)) {
+            best = tile;
+            best_of = of;
+        } else if (of < min_of) {
+            min_of = of;
+            optimal = tile;
+        }
+    }
+    if (best > 0) optimal = best;
+    return optimal > 0 ? optimal : 1;
+}
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* Indexing: indexing.h:39-73, indexing.cpp:5-110                            */

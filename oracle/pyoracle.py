@@ -176,22 +176,24 @@ def host_has_x86_64_v4() -> bool:
     return False
 
 
-def lib_path(strict: bool = True) -> str:
+def lib_path(strict: bool = True, iteration: str = "2d") -> str:
     """The strict checker, or the throughput build for this host's CPU (x86-64-v4 when the
-    host has it and the library was built, else x86-64-v3)."""
+    host has it and the library was built, else x86-64-v3) iterating as the reference's
+    USE_2D_ITER build (``iteration="2d"``) or its USE_MD_ITER + BALANCE_MD_ITER build ("md")."""
     if strict:
         return os.path.join(_HERE, "lib", "liboracle.so")
-    v4 = os.path.join(_HERE, "lib", "liboracle_fast_v4.so")
+    stem = {"2d": "liboracle_fast", "md": "liboracle_fast_md"}[iteration]
+    v4 = os.path.join(_HERE, "lib", stem + "_v4.so")
     if host_has_x86_64_v4() and os.path.exists(v4):
         return v4
-    return os.path.join(_HERE, "lib", "liboracle_fast.so")
+    return os.path.join(_HERE, "lib", stem + ".so")
 
 
-def load(strict: bool = True):
-    """Load (once) the strict checker or the fast-math baseline build."""
-    key = bool(strict)
+def load(strict: bool = True, iteration: str = "2d"):
+    """Load (once) the strict checker or a fast-math baseline build."""
+    key = (bool(strict), "strict" if strict else iteration)
     if key not in _libs:
-        path = lib_path(strict)
+        path = lib_path(strict, iteration)
         if not os.path.exists(path):
             raise FileNotFoundError(f"oracle library missing: {path} (run `make -C oracle`)")
         _libs[key] = _proto(C.CDLL(path))

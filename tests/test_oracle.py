@@ -10,6 +10,7 @@ No test re-types an oracle formula.
 """
 import ctypes as C
 import math
+import os
 import random
 from fractions import Fraction
 
@@ -594,3 +595,19 @@ def test_constant_time_step(po, oracle):
     dts, _, _ = _stepped_run(po, oracle, p)
     assert all(dt == Dt for dt in dts)
     assert len(dts) == int(np.ceil(p.max_time / Dt)) == 205
+
+
+def test_md_iteration_build(po):
+    """The throughput build iterating as the reference's USE_MD_ITER + BALANCE_MD_ITER build (row
+    tiles balanced over the threads) covers the same cells as the 2D-iteration build: the Sod KAT
+    takes the reference's 45 cycles and both builds' states agree to fast-math rounding."""
+    md = po.load(strict=False, iteration="md")
+    fast = po.load(strict=False)
+    assert md is not fast and "md" in os.path.basename(po.lib_path(False, "md"))
+    out = {}
+    for name, lib in (("md", md), ("2d", fast)):
+        p = po.reference_params(lib, "Sod")
+        st, d, cycles, _, _ = po.run(lib, p)
+        assert st == 0 and cycles == 45, (name, cycles)
+        out[name] = d.array("rho")[d.array("domain_mask") == 1].copy()
+    assert np.max(np.abs(out["md"] - out["2d"])) <= 1e-12 * np.max(np.abs(out["2d"]))
