@@ -50,6 +50,31 @@ template <class T> ARMON_HD T dv(T a, T b) { return a / b; }
 template <> ARMON_HD float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 #endif
 
+// x / d from inv = RN(1/d), computed once per cell size: the quotient q = RN(x * inv) and one
+// correction with the exact remainder, RN(q - RN(d * q - x) * inv). This is the correctly
+// rounded quotient RN(x / d) for every x that recip_div_ok accepts and every d in
+// [2^-100, 2^100] (Markstein's correction theorem: inv is within half an ulp of 1/d; when
+// |x| >= |d| the first quotient is faithful, and when |x| < |d| it is within 1.5 ulp and the
+// correction still cannot cross the midpoint nearest to x / d, since a quotient of two
+// 53-bit numbers lies at least 2^-106 / d from any midpoint; the remainder is exact or, in the
+// few corner cases near |x| = |d| = 2^k, within the same bound). A zero x gives the signed zero
+// of x / d. The caller checks x with recip_div_ok and divides where it fails.
+ARMON_HD double div_by_recip(double x, double d, double inv)
+{
+    const double q = x * inv;
+    const double r = __builtin_fma(d, q, -x);
+    return __builtin_fma(-r, inv, q);
+}
+
+// x is zero or its exponent lies in [-900, 900]: for d in [2^-100, 2^100] the remainder of
+// div_by_recip cannot underflow and the quotient cannot overflow.
+ARMON_HD bool recip_div_ok(double x)
+{
+    const uint64_t bits = (uint64_t) __builtin_bit_cast(long long, x);
+    const uint32_t e = (uint32_t) (bits >> 52) & 0x7FFu;
+    return x == 0 || e - (1023u - 900u) <= 1800u;
+}
+
 template <class T> ARMON_HD T sqrt_t(T x);
 template <> ARMON_HD double sqrt_t<double>(double x) { return __builtin_sqrt(x); }
 template <> ARMON_HD float sqrt_t<float>(float x) { return __builtin_sqrtf(x); }
@@ -222,10 +247,14 @@ ARMON_HD void advection_flux_2(T dx, T dt, T disp, T Dx,
     f_Erho = disp * (rho_c * E_c - slope_Er * length_factor);
 }
 
+// How the projection divides by dx: DX_DIVIDE (x / dx), DX_POW2 (dx is a power of two and
+// inv_dx = 1 / dx, so x / dx == x * inv_dx exactly), DX_RECIP (inv_dx = RN(1 / dx) with dx in
+// [2^-100, 2^100]: div_by_recip, the correctly rounded quotient, for the numerators that
+// recip_div_ok accepts; FP64 only).
+enum DxMode : int { DX_DIVIDE = 0, DX_POW2 = 1, DX_RECIP = 2 };
+
 // euler_projection.cpp:42-52: remap of cell i from the advection fluxes at faces i, i+1.
-// `inv_dx` is used instead of a division only when dx is a power of two, where
-// x / dx == x * (1/dx) holds exactly; pass DX_POW2 = false otherwise.
-template <bool DX_POW2, class T>
+template <int DXM, class T>
 ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
                          T f_rho_i, T f_rho_ip, T f_urho_i, T f_urho_ip,
                          T f_vrho_i, T f_vrho_ip, T f_Erho_i, T f_Erho_ip,
@@ -233,11 +262,27 @@ ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
 {
     T dX = dx + dt * (us_ip - us_i);
     T n_rho, n_urho, n_vrho, n_Erho;
-    if constexpr (DX_POW2) {
+    if constexpr (DXM == DX_POW2) {
         n_rho = (dX * rho - (f_rho_ip - f_rho_i)) * inv_dx;
         n_urho = (dX * rho * u - (f_urho_ip - f_urho_i)) * inv_dx;
         n_vrho = (dX * rho * v - (f_vrho_ip - f_vrho_i)) * inv_dx;
         n_Erho = (dX * rho * E - (f_Erho_ip - f_Erho_i)) * inv_dx;
+    } else if constexpr (DXM == DX_RECIP && sizeof(T) == 8) {
+        const T a_rho = dX * rho - (f_rho_ip - f_rho_i);
+        const T a_urho = dX * rho * u - (f_urho_ip - f_urho_i);
+        const T a_vrho = dX * rho * v - (f_vrho_ip - f_vrho_i);
+        const T a_Erho = dX * rho * E - (f_Erho_ip - f_Erho_i);
+        if (recip_div_ok(a_rho) & recip_div_ok(a_urho) & recip_div_ok(a_vrho) & recip_div_ok(a_Erho)) {
+            n_rho = div_by_recip(a_rho, dx, inv_dx);
+            n_urho = div_by_recip(a_urho, dx, inv_dx);
+            n_vrho = div_by_recip(a_vrho, dx, inv_dx);
+            n_Erho = div_by_recip(a_Erho, dx, inv_dx);
+        } else {
+            n_rho = dv<T>(a_rho, dx);
+            n_urho = dv<T>(a_urho, dx);
+            n_vrho = dv<T>(a_vrho, dx);
+            n_Erho = dv<T>(a_Erho, dx);
+        }
     } else {
         (void) inv_dx;
         n_rho = dv<T>(dX * rho - (f_rho_ip - f_rho_i), dx);

@@ -243,6 +243,103 @@ __global__ void __launch_bounds__(kBlock) k_acoustic_GAD(Iter2D it, int64_t s, f
     });
 }
 
+// The GAD flux of acoustic.cpp:40-88 solves the Godunov problem at faces i-s, i and i+s for
+// every face i, so each face's solution is computed three times. The two kernels below solve
+// it once per face and hand it to the neighbouring faces (same arithmetic, same bits):
+//  - along rows (s = 1), through LDS: a block covers kGadTile consecutive faces of one row and
+//    writes the kGadTile - 2 inner ones;
+//  - across rows (s = the row step), in registers: each thread marches down a run of rows of
+//    its column with the last three solutions in a sliding window.
+// A face's solution is computed only where an active face (one of the range) or one of its
+// two neighbours needs it, so the kernels read nothing the generic kernel would not.
+constexpr int kGadTile = 256;
+constexpr int kGadOut = kGadTile - 2;
+
+// Active columns [lo, hi) of `row` of the range (a partial first or last row).
+__device__ __forceinline__ void row_columns(const Iter2D& it, int64_t row, int64_t& lo, int64_t& hi)
+{
+    lo = row == it.first_row ? it.start - row * it.len : 0;
+    hi = row == it.last_row ? it.end - row * it.len : it.len;
+}
+
+template <int L>
+__global__ void __launch_bounds__(kGadTile) k_acoustic_GAD_x(Iter2D it, flt_t dt, flt_t dx,
+                                                            flt_t* __restrict__ ustar, flt_t* __restrict__ pstar,
+                                                            const flt_t* __restrict__ rho,
+                                                            const flt_t* __restrict__ u,
+                                                            const flt_t* __restrict__ pmat,
+                                                            const flt_t* __restrict__ cmat)
+{
+    __shared__ flt_t g_u[kGadTile], g_p[kGadTile];
+    const int t = threadIdx.x;
+    const int64_t col = (int64_t) blockIdx.x * kGadOut - 1 + t;   // this thread's face
+    for (int64_t row = it.first_row + blockIdx.y; row <= it.last_row; row += gridDim.y) {
+        int64_t lo, hi;
+        row_columns(it, row, lo, hi);
+        const int64_t i = it.base + row * it.step + col;
+        flt_t r_c = 0, r_m = 0, c_c = 0, c_m = 0, u_c = 0, u_m = 0, p_c = 0, p_m = 0;
+        if (col >= lo - 1 && col <= hi) {
+            r_c = rho[i]; r_m = rho[i - 1];
+            c_c = cmat[i]; c_m = cmat[i - 1];
+            u_c = u[i]; u_m = u[i - 1];
+            p_c = pmat[i]; p_m = pmat[i - 1];
+            godunov<flt_t>(r_c, r_m, c_c, c_m, u_c, u_m, p_c, p_m, g_u[t], g_p[t]);
+        }
+        __syncthreads();
+        if (t >= 1 && t <= kGadOut && col >= lo && col < hi) {
+            flt_t us, ps;
+            gad_correct<L, flt_t>(dt, dx, r_m, r_c, c_m, c_c, u_m, u_c, p_m, p_c, g_u[t - 1], g_p[t - 1], g_u[t],
+                                  g_p[t], g_u[t + 1], g_p[t + 1], us, ps);
+            ustar[i] = us;
+            pstar[i] = ps;
+        }
+        __syncthreads();   // the next row reuses the LDS
+    }
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBlock) k_acoustic_GAD_y(Iter2D it, int64_t run, flt_t dt, flt_t dx,
+                                                           flt_t* __restrict__ ustar, flt_t* __restrict__ pstar,
+                                                           const flt_t* __restrict__ rho, const flt_t* __restrict__ u,
+                                                           const flt_t* __restrict__ pmat,
+                                                           const flt_t* __restrict__ cmat)
+{
+    const int64_t col = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= it.len) return;
+    // rows of this column in the range: [r_lo, r_hi]
+    const int64_t r_lo = it.first_row + (col < it.start - it.first_row * it.len ? 1 : 0);
+    const int64_t r_hi = it.last_row - (col > it.end - 1 - it.last_row * it.len ? 1 : 0);
+    const int64_t s = it.step;
+    const int64_t nruns = (it.last_row - it.first_row + run) / run;
+    for (int64_t k = blockIdx.y; k < nruns; k += gridDim.y) {
+        const int64_t ra = max(it.first_row + k * run, r_lo);
+        const int64_t rb = min(it.first_row + (k + 1) * run - 1, r_hi);
+        if (ra > rb) continue;
+        // cells of rows ra - 2 (mm), ra - 1 (m), and the solutions at faces ra - 1 and ra
+        int64_t i = it.base + ra * s + col;
+        flt_t r_mm = rho[i - 2 * s], c_mm = cmat[i - 2 * s], u_mm = u[i - 2 * s], p_mm = pmat[i - 2 * s];
+        flt_t r_m = rho[i - s], c_m = cmat[i - s], u_m = u[i - s], p_m = pmat[i - s];
+        flt_t r_c = rho[i], c_c = cmat[i], u_c = u[i], p_c = pmat[i];
+        flt_t gu_m, gp_m, gu_c, gp_c;
+        godunov<flt_t>(r_m, r_mm, c_m, c_mm, u_m, u_mm, p_m, p_mm, gu_m, gp_m);
+        godunov<flt_t>(r_c, r_m, c_c, c_m, u_c, u_m, p_c, p_m, gu_c, gp_c);
+        for (int64_t row = ra; row <= rb; row++, i += s) {
+            const flt_t r_p = rho[i + s], c_p = cmat[i + s], u_p = u[i + s], p_p = pmat[i + s];
+            flt_t gu_p, gp_p;
+            godunov<flt_t>(r_p, r_c, c_p, c_c, u_p, u_c, p_p, p_c, gu_p, gp_p);
+            flt_t us, ps;
+            gad_correct<L, flt_t>(dt, dx, r_m, r_c, c_m, c_c, u_m, u_c, p_m, p_c, gu_m, gp_m, gu_c, gp_c, gu_p, gp_p,
+                                  us, ps);
+            ustar[i] = us;
+            pstar[i] = ps;
+            r_m = r_c; c_m = c_c; u_m = u_c; p_m = p_c;
+            r_c = r_p; c_c = c_p; u_c = u_p; p_c = p_p;
+            gu_m = gu_c; gp_m = gp_c;
+            gu_c = gu_p; gp_c = gp_p;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_perfect_gas_EOS(Iter2D it, flt_t gamma, const flt_t* __restrict__ rho,
                                                             const flt_t* __restrict__ Emat,
                                                             const flt_t* __restrict__ umat,
@@ -286,7 +383,10 @@ __global__ void __launch_bounds__(kBlock) k_cell_update(Iter2D it, int64_t s, fl
     });
 }
 
-__global__ void __launch_bounds__(kBlock) k_euler_projection(Iter2D it, int64_t s, flt_t dx, flt_t dt,
+// DXM: how x / dx is computed (DxMode), with inv_dx = 1 / dx for DX_POW2 and DX_RECIP; every mode
+// gives the IEEE quotient's bits.
+template <int DXM>
+__global__ void __launch_bounds__(kBlock) k_euler_projection(Iter2D it, int64_t s, flt_t dx, flt_t inv_dx, flt_t dt,
                                                              const flt_t* __restrict__ ustar, flt_t* __restrict__ rho,
                                                              flt_t* __restrict__ umat, flt_t* __restrict__ vmat,
                                                              flt_t* __restrict__ Emat, const flt_t* __restrict__ a_rho,
@@ -296,8 +396,8 @@ __global__ void __launch_bounds__(kBlock) k_euler_projection(Iter2D it, int64_t 
 {
     for_each_2d(it, [&](int64_t i) {
         flt_t r = rho[i], u = umat[i], v = vmat[i], E = Emat[i];
-        projection<false, flt_t>(dx, flt_t(0), dt, ustar[i], ustar[i + s], a_rho[i], a_rho[i + s], a_urho[i],
-                                 a_urho[i + s], a_vrho[i], a_vrho[i + s], a_Erho[i], a_Erho[i + s], r, u, v, E);
+        projection<DXM, flt_t>(dx, inv_dx, dt, ustar[i], ustar[i + s], a_rho[i], a_rho[i + s], a_urho[i],
+                               a_urho[i + s], a_vrho[i], a_vrho[i + s], a_Erho[i], a_Erho[i + s], r, u, v, E);
         rho[i] = r;
         umat[i] = u;
         vmat[i] = v;
@@ -690,11 +790,31 @@ ARMON_KERNEL_TRY
     check_view(pmat, "pmat", sp_n);
     check_view(cmat, "cmat", sp_n);
     if (empty_2d(it)) return;
-    const dim3 g = grid_2d(it);
+    const int64_t rows = it.last_row - it.first_row + 1;
+    auto launch = [&]<int LL>() {
+        if (s == 1) {
+            // one block per kGadOut columns of a row
+            const dim3 g((unsigned) ((it.len + kGadOut - 1) / kGadOut), (unsigned) (rows < 65535 ? rows : 65535));
+            k_acoustic_GAD_x<LL><<<g, kGadTile, 0, stream()>>>(it, dt, dx, ustar.data, pstar.data, rho.data, u.data,
+                                                               pmat.data, cmat.data);
+        } else if (s == it.step) {
+            // runs of up to 64 rows, shorter while that leaves fewer than 8192 waves of 64 columns
+            int64_t run = 64;
+            const int64_t groups = (it.len + 63) / 64;
+            while (run > 8 && groups * ((rows + run - 1) / run) < 8192) run /= 2;
+            const int64_t nruns = (rows + run - 1) / run;
+            const dim3 g((unsigned) ((it.len + kBlock - 1) / kBlock), (unsigned) (nruns < 65535 ? nruns : 65535));
+            k_acoustic_GAD_y<LL><<<g, kBlock, 0, stream()>>>(it, run, dt, dx, ustar.data, pstar.data, rho.data,
+                                                             u.data, pmat.data, cmat.data);
+        } else {
+            k_acoustic_GAD<LL><<<grid_2d(it), kBlock, 0, stream()>>>(it, s, dt, dx, ustar.data, pstar.data, rho.data,
+                                                                      u.data, pmat.data, cmat.data);
+        }
+    };
     switch (L) {
-    case 0: k_acoustic_GAD<0><<<g, kBlock, 0, stream()>>>(it, s, dt, dx, ustar.data, pstar.data, rho.data, u.data, pmat.data, cmat.data); break;
-    case 1: k_acoustic_GAD<1><<<g, kBlock, 0, stream()>>>(it, s, dt, dx, ustar.data, pstar.data, rho.data, u.data, pmat.data, cmat.data); break;
-    default: k_acoustic_GAD<2><<<g, kBlock, 0, stream()>>>(it, s, dt, dx, ustar.data, pstar.data, rho.data, u.data, pmat.data, cmat.data); break;
+    case 0: launch.template operator()<0>(); break;
+    case 1: launch.template operator()<1>(); break;
+    default: launch.template operator()<2>(); break;
     }
     after_launch("acoustic_GAD");
 }
@@ -780,10 +900,20 @@ ARMON_KERNEL_TRY
     check_view(advection_vrho, "advection_vrho", sp_f, false);
     check_view(advection_Erho, "advection_Erho", sp_f, false);
     if (empty_2d(it)) return;
-    k_euler_projection<<<grid_2d(it), kBlock, 0, stream()>>>(it, s, dx, dt, ustar.data, rho.data, umat.data,
-                                                              vmat.data, Emat.data, advection_rho.data,
-                                                              advection_urho.data, advection_vrho.data,
-                                                              advection_Erho.data);
+    // the division by dx: a multiply when dx is a power of two, else the reciprocal-corrected
+    // quotient (device_math.hpp), both equal to the IEEE quotient
+    int e = 0;
+    const bool pow2 = dx > 0 && std::isfinite(dx) && std::frexp(dx, &e) == 0.5;
+    const bool recip = !pow2 && dx >= 0x1p-100 && dx <= 0x1p100;
+    const flt_t inv_dx = (pow2 || recip) ? flt_t(1) / dx : flt_t(0);
+    auto launch = [&](auto kernel) {
+        kernel<<<grid_2d(it), kBlock, 0, stream()>>>(it, s, dx, inv_dx, dt, ustar.data, rho.data, umat.data, vmat.data,
+                                                     Emat.data, advection_rho.data, advection_urho.data,
+                                                     advection_vrho.data, advection_Erho.data);
+    };
+    if (pow2) launch(k_euler_projection<DX_POW2>);
+    else if (recip) launch(k_euler_projection<DX_RECIP>);
+    else launch(k_euler_projection<DX_DIVIDE>);
     after_launch("euler_projection");
 }
 ARMON_KERNEL_CATCH("euler_projection")

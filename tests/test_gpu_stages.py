@@ -122,6 +122,42 @@ def test_acoustic_GAD(armon, po, oracle, axis, limiter):
 
 
 @pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("cut", ["full", "partial"])
+def test_acoustic_GAD_shared_faces(armon, po, oracle, axis, cut):
+    """The GAD flux kernels that solve each face once (LDS tiles of 254 faces along x, runs of
+    rows along y): a domain of several tiles and runs, over the whole flux range or a range cut in
+    the middle of its first and last rows, bit for bit against the oracle."""
+    lib = armon.load()
+    p = domain(armon, axis=axis, nx=600, ny=300)
+    names = ["ustar", "pstar", "rho", "u", "pmat", "cmat"]
+    c = Case(armon, p.nb_cells, names, seed=SEED + 11 + axis)
+    r, ir, orr, oir = it2d(armon, po, lib.armon_domain_fluxes(C.byref(p)))
+    if cut == "partial":
+        n = ir.row_range_length
+        r.start, r.end = n // 3 + 5 * n, r.end - n // 2 - 3 * n - 7
+        orr.start, orr.end = r.start, r.end
+    print(f"case acoustic_GAD axis {axis} range {r.start}..{r.end} of rows of {ir.row_range_length}", flush=True)
+    dt, dx = 0.0123, p.dx
+    lib.acoustic_GAD(C.byref(r), C.byref(ir), p.s, dt, dx, *[c.v(k) for k in names], 2)
+    assert oracle.o_acoustic_GAD(C.byref(orr), C.byref(oir), p.s, dt, dx, *[c.h(k) for k in names], 2) == 0
+    c.compare(["ustar", "pstar"])
+
+
+def test_acoustic_GAD_other_stride(armon, po, oracle):
+    """A stride that is neither 1 nor the row step goes through the generic kernel (three solves
+    per face, as acoustic.cpp:65-86): s = 2 over the x flux range, bit for bit."""
+    lib = armon.load()
+    p = domain(armon, axis=0, nx=60, ny=20)
+    names = ["ustar", "pstar", "rho", "u", "pmat", "cmat"]
+    c = Case(armon, p.nb_cells, names, seed=SEED + 21)
+    r, ir, orr, oir = it2d(armon, po, lib.armon_domain_fluxes(C.byref(p)))
+    print("case acoustic_GAD s=2", flush=True)
+    lib.acoustic_GAD(C.byref(r), C.byref(ir), 2, 0.0123, p.dx, *[c.v(k) for k in names], 1)
+    assert oracle.o_acoustic_GAD(C.byref(orr), C.byref(oir), 2, 0.0123, p.dx, *[c.h(k) for k in names], 1) == 0
+    c.compare(["ustar", "pstar"])
+
+
+@pytest.mark.parametrize("axis", [0, 1])
 def test_cell_update(armon, po, oracle, axis):
     lib = armon.load()
     p = domain(armon, axis=axis)
