@@ -107,7 +107,7 @@ class ArmonDomainSpec(C.Structure):
 
 class ArmonBatchTuning(C.Structure):
     _fields_ = [("y_run_rows", C.c_int), ("grid_x", C.c_int), ("grid_y", C.c_int), ("items_x", C.c_int),
-                ("items_y", C.c_int), ("ms_x", C.c_double), ("ms_y", C.c_double)]
+                ("items_y", C.c_int), ("ms_x", C.c_double), ("ms_y", C.c_double), ("tile_x", C.c_int)]
 
 
 class ArmonKernelTimes(C.Structure):
@@ -192,6 +192,9 @@ EXTENSION_SYMBOLS = {
     "armon_solver_advance": (C.c_int, [C.c_void_p, C.c_int]),
     "armon_solver_finish": (C.c_int, [C.c_void_p, C.POINTER(ArmonResult)]),
     "armon_solver_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ArmonKernelTimes)]),
+    "armon_solver_stage_count": (C.c_int, [C.c_void_p]),
+    "armon_solver_stage_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(_d),
+                                          C.POINTER(C.c_int64)]),
     "armon_solver_read_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_d), C.c_int64]),
     "armon_solver_write_field": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_d), C.c_int64]),
     "armon_solver_conservation": (C.c_int, [C.c_void_p, C.POINTER(_d), C.POINTER(_d)]),
@@ -377,6 +380,17 @@ class Solver:
         _check(load().armon_solver_kernel_times(self.h, int(n), C.byref(t)), "kernel_times")
         return t
 
+    def stage_times(self) -> dict:
+        """{label: (ms, calls)} of the per-stage device times (recorded when verbose < 3)."""
+        lib = load()
+        out = {}
+        for k in range(lib.armon_solver_stage_count(self.h)):
+            label, ms, calls = C.c_char_p(), C.c_double(), C.c_int64()
+            _check(lib.armon_solver_stage_time(self.h, k, C.byref(label), C.byref(ms), C.byref(calls)),
+                   "stage_time")
+            out[label.value.decode()] = (ms.value, calls.value)
+        return out
+
     def read(self, field) -> np.ndarray:
         f = FIELD[field] if isinstance(field, str) else int(field)
         out = np.empty(self.n, dtype=np.float64)
@@ -475,8 +489,8 @@ class Batch:
         load().armon_batch_get_tuning(self.h, C.byref(t))
         return t
 
-    def set_tuning(self, y_run_rows: int, grid_x: int = 0, grid_y: int = 0):
-        t = ArmonBatchTuning(y_run_rows=y_run_rows, grid_x=grid_x, grid_y=grid_y)
+    def set_tuning(self, y_run_rows: int, grid_x: int = 0, grid_y: int = 0, tile_x: int = 0):
+        t = ArmonBatchTuning(y_run_rows=y_run_rows, grid_x=grid_x, grid_y=grid_y, tile_x=tile_x)
         _check(load().armon_batch_set_tuning(self.h, C.byref(t)), "armon_batch_set_tuning")
 
     def begin(self):

@@ -11,6 +11,7 @@
 // single-domain fused sweeps, so its result is the single-domain result.
 #include "internal.hpp"
 #include "fused.hpp"
+#include "device_math.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -29,6 +30,7 @@ namespace {
 constexpr int64_t kAlign = 64;   // domain slices start on 64-element boundaries
 constexpr int kWave = 64;
 constexpr int kYWavesPerBlock = 4;
+constexpr int kBatchTileWidths[3] = {128, 256, 512};   // X tile widths the autotuner tries
 
 uint64_t splitmix64(uint64_t& s)
 {
@@ -85,8 +87,8 @@ struct ArmonBatch {
     int sweeps_per_cycle = 2;
     int64_t cycles_enqueued = 0;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
+    hipGraphExec_t graph[2] = {};        // graph_chunk cycles from an even (0) or odd (1) cycle
+    hipGraph_t graph_def[2] = {};
     bool initialized = false, begun = false, finished = false;
     std::vector<DeviceState> final_states;
 
@@ -100,8 +102,10 @@ struct ArmonBatch {
 
 ArmonBatch::~ArmonBatch()
 {
-    if (graph) (void) hipGraphExecDestroy(graph);
-    if (graph_def) (void) hipGraphDestroy(graph_def);
+    for (int k = 0; k < 2; k++) {
+        if (graph[k]) (void) hipGraphExecDestroy(graph[k]);
+        if (graph_def[k]) (void) hipGraphDestroy(graph_def[k]);
+    }
     for (auto* a : arr)
         if (a) (void) hipFree(a);
     for (auto* a : f32)
@@ -126,9 +130,21 @@ void fail(const char* where, const std::exception& e)
 // neighbouring column groups of one band (measured 2% faster in FP64 and 12% in FP32-mixed
 // than column group by column group); then longest runs first (a stable sort) so that a
 // grid-stride walk over them balances well.
+// Destroys the captured graphs: they hold the work lists' addresses and the launch grids.
+void drop_graphs(ArmonBatch* b)
+{
+    for (int k = 0; k < 2; k++) {
+        if (b->graph[k]) (void) hipGraphExecDestroy(b->graph[k]);
+        if (b->graph_def[k]) (void) hipGraphDestroy(b->graph_def[k]);
+        b->graph[k] = nullptr;
+        b->graph_def[k] = nullptr;
+    }
+}
+
 void build_items(ArmonBatch* b, int run)
 {
-    const int W = x_tile_outputs(b->var);
+    drop_graphs(b);
+    const int W = x_tile_outputs(b->var, b->tuning.tile_x);
     const int R = read_extent(b->var.flux, b->var.proj);
     b->items_x.clear();
     b->items_y.clear();
@@ -217,10 +233,10 @@ void launch_one_sweep(ArmonBatch* b, int axis, int parity, double factor, bool r
     const int grid = x ? blocks_x(b, b->tuning.grid_x) : blocks_y(b, b->tuning.grid_y);
     if (b->fp64())
         launch_batch_sweep<double>(axis, b->var, batch_args<double>(b, parity, factor, reduce, states), items, nitems,
-                                   grid, b->stream);
+                                   grid, b->tuning.tile_x, b->stream);
     else
         launch_batch_sweep<float>(axis, b->var, batch_args<float>(b, parity, factor, reduce, states), items, nitems,
-                                  grid, b->stream);
+                                  grid, b->tuning.tile_x, b->stream);
 }
 
 struct Timer {
@@ -269,31 +285,35 @@ void enqueue_cycle(ArmonBatch* b, int64_t cycle, Timer* t)
     }
 }
 
-void build_graph(ArmonBatch* b)
+// graph_chunk cycles from a cycle of parity `parity`: a cycle's launches depend on its index
+// only through its parity (splitting order, buffer parity), so the graph replays any run of
+// graph_chunk cycles that starts at that parity.
+void build_graph(ArmonBatch* b, int parity)
 {
     hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-        for (int c = 0; c < b->common.graph_chunk; c++) enqueue_cycle(b, c, nullptr);
+        for (int c = 0; c < b->common.graph_chunk; c++) enqueue_cycle(b, parity + c, nullptr);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void) hipStreamEndCapture(b->stream, &g);
         if (g) (void) hipGraphDestroy(g);
         throw;
     }
-    hip_check(hipStreamEndCapture(b->stream, &b->graph_def), "hipStreamEndCapture");
-    hip_check(hipGraphInstantiate(&b->graph, b->graph_def, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hip_check(hipStreamEndCapture(b->stream, &b->graph_def[parity]), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&b->graph[parity], b->graph_def[parity], nullptr, nullptr, 0),
+              "hipGraphInstantiate");
 }
 
-// A graph of `chunk` cycles recorded from cycle 0 replays any chunk starting at a multiple
-// of chunk: chunk is even, so the splitting order and the buffer parity are the same.
+// n more cycles: graph launches while at least graph_chunk cycles remain, then one by one.
 void advance(ArmonBatch* b, int64_t n)
 {
     const int chunk = b->common.graph_chunk;
     while (n > 0) {
         const int64_t c = b->cycles_enqueued;
-        if (chunk > 0 && chunk % 2 == 0 && c % chunk == 0 && n >= chunk) {
-            if (!b->graph) build_graph(b);
-            hip_check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
+        if (chunk > 0 && n >= chunk) {
+            const int parity = (int) (c & 1);
+            if (!b->graph[parity]) build_graph(b, parity);
+            hip_check(hipGraphLaunch(b->graph[parity], b->stream), "hipGraphLaunch");
             b->cycles_enqueued += chunk;
             n -= chunk;
         } else {
@@ -482,8 +502,10 @@ ARMON_EXPORT ArmonBatch* armon_batch_create(const ArmonParams* common, int n, co
             D.dx[0] = q.domain_size[0] / (double) q.nx;
             D.dx[1] = q.domain_size[1] / (double) q.ny;
             for (int a = 0; a < 2; a++) {
-                D.dx_pow2[a] = pow2(D.dx[a]) ? 1 : 0;
-                D.inv_dx[a] = D.dx_pow2[a] ? 1.0 / D.dx[a] : 0.0;
+                // the batched sweeps divide by dx through its reciprocal (DX_RECIP)
+                if (!(D.dx[a] >= 0x1p-100 && D.dx[a] <= 0x1p100))
+                    throw invalid_argument("domain " + std::to_string(d) + ": cell size outside [2^-100, 2^100]");
+                D.inv_dx[a] = 1.0 / D.dx[a];
             }
             D.cfl_dx = D.dx[0];
             D.cfl_dy = D.dx[1];
@@ -498,7 +520,8 @@ ARMON_EXPORT ArmonBatch* armon_batch_create(const ArmonParams* common, int n, co
         b->d_tune = device_alloc<DeviceState>(n, "hipMalloc(tune states)");
         hip_check(hipMemcpy(b->d_domains, dom.data(), sizeof(BatchDomain) * n, hipMemcpyHostToDevice), "upload domains");
         hip_check(hipMemcpy(b->d_cfgs, cfg.data(), sizeof(TimeStepConfig) * n, hipMemcpyHostToDevice), "upload configs");
-        // default configuration: 256-row Y runs, one block per work item
+        // default configuration: 256-thread X tiles, 256-row Y runs, one block per work item
+        b->tuning.tile_x = 256;
         b->tuning.grid_x = 0;
         b->tuning.grid_y = 0;
         build_items(b, 256);
@@ -567,6 +590,9 @@ ARMON_EXPORT int armon_batch_set_tuning(ArmonBatch* b, const ArmonBatchTuning* t
     try {
         check_ready(b, false, true);
         if (t->y_run_rows < 1) throw invalid_argument("y_run_rows must be positive");
+        if (t->tile_x != 0 && t->tile_x != 128 && t->tile_x != 256 && t->tile_x != 512)
+            throw invalid_argument("tile_x must be 128, 256 or 512 (0: 256)");
+        b->tuning.tile_x = t->tile_x ? t->tile_x : 256;
         b->tuning.grid_x = std::max(0, t->grid_x);
         b->tuning.grid_y = std::max(0, t->grid_y);
         build_items(b, t->y_run_rows);
@@ -594,21 +620,28 @@ ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out)
         hip_check(hipMemcpyAsync(b->d_tune, tune.data(), sizeof(DeviceState) * tune.size(), hipMemcpyHostToDevice,
                                  b->stream),
                   "upload tune states");
-        const int rx = b->fp64() ? batch_resident_blocks<double>(ARMON_AXIS_X, b->var)
-                                 : batch_resident_blocks<float>(ARMON_AXIS_X, b->var);
-        const int ry = b->fp64() ? batch_resident_blocks<double>(ARMON_AXIS_Y, b->var)
-                                 : batch_resident_blocks<float>(ARMON_AXIS_Y, b->var);
+        const int ry = b->fp64() ? batch_resident_blocks<double>(ARMON_AXIS_Y, b->var, 256)
+                                 : batch_resident_blocks<float>(ARMON_AXIS_Y, b->var, 256);
         const int reps = 5;
 
-        // X: persistent grids of 1, 2, 4 resident rounds, or one block per tile
+        // X: tile width (128, 256, 512 threads, one cell each) x (persistent grids of 1, 2, 4
+        // resident rounds | one block per tile)
         double best_x = 1e300;
-        int grid_x = 0;
-        for (int g : {rx, 2 * rx, 4 * rx, 0}) {
-            if (g < 0) continue;
-            b->tuning.grid_x = g;
-            const double ms = time_sweep(b, ARMON_AXIS_X, reps);
-            if (ms < best_x) { best_x = ms; grid_x = g; }
+        int grid_x = 0, tile_x = 256;
+        const int run0 = b->tuning.y_run_rows;
+        for (int tile : kBatchTileWidths) {
+            b->tuning.tile_x = tile;
+            build_items(b, run0);
+            const int rx = b->fp64() ? batch_resident_blocks<double>(ARMON_AXIS_X, b->var, tile)
+                                     : batch_resident_blocks<float>(ARMON_AXIS_X, b->var, tile);
+            for (int g : {rx, 2 * rx, 4 * rx, 0}) {
+                if (g < 0) continue;
+                b->tuning.grid_x = g;
+                const double ms = time_sweep(b, ARMON_AXIS_X, reps);
+                if (ms < best_x) { best_x = ms; grid_x = g; tile_x = tile; }
+            }
         }
+        b->tuning.tile_x = tile_x;
         b->tuning.grid_x = grid_x;
 
         // Y: run length x (one resident round of blocks | one block per 4 runs)
@@ -670,6 +703,10 @@ ARMON_EXPORT int armon_batch_begin(ArmonBatch* b)
         }
         hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         b->cycles_enqueued = 0;
+        // both graphs now, so that no later advance pays for a capture
+        if (b->common.graph_chunk > 0)
+            for (int parity = 0; parity < 2; parity++)
+                if (!b->graph[parity]) build_graph(b, parity);
         b->begun = true;
         b->finished = false;
         return 0;

@@ -8,6 +8,7 @@
 #include "internal.hpp"
 #include "fused.hpp"
 #include "device_math.hpp"
+#include "trace.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -18,6 +19,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <iterator>
 #include <limits>
 #include <string>
 #include <vector>
@@ -138,14 +140,15 @@ struct ArmonSolver {
     float* f32[10] = {};             // FP32-mixed buffers: A rho,u,v,E | B rho,u,v,E | stale p, c
     int sweeps_per_cycle = 2;
     int64_t cycles_enqueued = 0;
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
+    hipGraphExec_t graph[2] = {};    // graph_chunk cycles from an even (0) or odd (1) cycle
+    hipGraph_t graph_def[2] = {};
     bool begun = false;
     double initial_mass = 0, initial_energy = 0;
     std::chrono::steady_clock::time_point t_begin;
     int error_count_at_begin = 0;
     bool dt_error = false;           // an invalid time step stopped the loop (armon_2D.cpp:238-242)
     bool compare_stop = false;       // a per-step comparison found a difference
+    StageClock clock;                // per-stage device time (on when verbose < 3) and roctx ranges
 
     ~ArmonSolver();
 };
@@ -308,23 +311,28 @@ bool staged_time_step(ArmonSolver* s)
 bool staged_solver_cycle(ArmonSolver* s)
 {
     ArmonParams& p = s->p;
-    if (!staged_time_step(s)) return false;
+    Stage cycle(s->clock, "cycle", nullptr);
+    {
+        Stage st(s->clock, "time_step", "dt_CFL");
+        if (!staged_time_step(s)) return false;
+    }
     if (step_checkpoint(s, "time_step")) return false;
     int axes[3];
     double factors[3];
     const int n = armon_split_axes(&p, axes, factors);
     for (int k = 0; k < n; k++) {
+        Stage axis(s->clock, "axis", nullptr);
         armon_update_axis(&p, axes[k]);
         p.cycle_dt = p.current_cycle_dt * factors[k];
-        staged_update_EOS(s);
+        { Stage st(s->clock, "EOS", "EOS"); staged_update_EOS(s); }
         if (step_checkpoint(s, "EOS")) return false;
-        staged_boundary_conditions(s);
+        { Stage st(s->clock, "BC", "boundary_conditions"); staged_boundary_conditions(s); }
         if (step_checkpoint(s, "boundary_conditions")) return false;
-        staged_numerical_fluxes(s);
+        { Stage st(s->clock, "fluxes", "numerical_fluxes"); staged_numerical_fluxes(s); }
         if (step_checkpoint(s, "numerical_fluxes")) return false;
-        staged_cell_update(s);
+        { Stage st(s->clock, "update", "cell_update"); staged_cell_update(s); }
         if (step_checkpoint(s, "cell_update")) return false;
-        staged_projection_remap(s);
+        { Stage st(s->clock, "remap", "euler_proj"); staged_projection_remap(s); }
         if (step_checkpoint(s, "projection_remap")) return false;
     }
     return true;
@@ -359,6 +367,9 @@ bool fused_supported(const ArmonParams& p, std::string& why)
     }
     if ((int64_t) p.nb_cells >= (int64_t) 1 << 31) { why = "more than 2^31 cells"; return false; }
     if (p.nx >= 1 << 23) { why = "rows of 2^23 cells or more"; return false; }   // X tiles' slot arithmetic
+    // the sweeps divide by the cell sizes through their reciprocals (DX_RECIP)
+    for (double d : {p.domain_size[0] / (double) p.nx, p.domain_size[1] / (double) p.ny})
+        if (!(d >= 0x1p-100 && d <= 0x1p100)) { why = "a cell size outside [2^-100, 2^100]"; return false; }
     return true;
 }
 
@@ -377,6 +388,7 @@ bool is_pow2(double x)
     int e;
     return std::frexp(x, &e) == 0.5;
 }
+
 
 template <class T>
 SweepArgs<T> sweep_args(ArmonSolver* s, int axis, double factor, bool reduce, T* const in[4], T* const out[4],
@@ -397,8 +409,8 @@ SweepArgs<T> sweep_args(ArmonSolver* s, int axis, double factor, bool reduce, T*
     a.seg_len = p.y_run_rows;   // > 0: rows per run; 0: the launcher chooses
     a.gamma = 1.4;
     a.dx = axis == ARMON_AXIS_X ? p.domain_size[0] / (double) p.nx : p.domain_size[1] / (double) p.ny;
-    a.dx_pow2 = is_pow2(a.dx) ? 1 : 0;
-    a.inv_dx = a.dx_pow2 ? 1.0 / a.dx : 0.0;
+    a.dx_mode = is_pow2(a.dx) ? DX_POW2 : DX_RECIP;
+    a.inv_dx = 1.0 / a.dx;
     a.dt_factor = factor;
     double lu, lv, hu, hv;
     boundary_factors(p, axis == ARMON_AXIS_X ? ARMON_SIDE_LEFT : ARMON_SIDE_BOTTOM, &lu, &lv);
@@ -473,40 +485,43 @@ void fused_enqueue_cycle(ArmonSolver* s, int64_t cycle, Timer* timer)
     }
 }
 
-void fused_build_graph(ArmonSolver* s)
+// Captures graph_chunk cycles starting at a cycle of parity `parity`. What a cycle enqueues
+// depends on its index only through its parity (the splitting order of parameters.cpp:212-255
+// and the ping-pong buffer parity), and everything else (dt, time, cycle count) lives on the
+// device, so the graph replays any run of graph_chunk cycles that starts at that parity.
+void fused_build_graph(ArmonSolver* s, int parity)
 {
     const int chunk = s->p.graph_chunk;
     hip_check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-        for (int c = 0; c < chunk; c++) fused_enqueue_cycle(s, c, nullptr);
+        for (int c = 0; c < chunk; c++) fused_enqueue_cycle(s, parity + c, nullptr);
     } catch (...) {
-        hipGraph_t g;
+        hipGraph_t g = nullptr;
         (void) hipStreamEndCapture(s->stream, &g);
         if (g) (void) hipGraphDestroy(g);
         throw;
     }
-    hip_check(hipStreamEndCapture(s->stream, &s->graph_def), "hipStreamEndCapture");
-    hip_check(hipGraphInstantiate(&s->graph, s->graph_def, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hip_check(hipStreamEndCapture(s->stream, &s->graph_def[parity]), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&s->graph[parity], s->graph_def[parity], nullptr, nullptr, 0),
+              "hipGraphInstantiate");
 }
 
-// A graph of `chunk` cycles recorded from cycle 0 is valid for any start cycle with the same
-// parity of cycle index and buffer: chunk is even, so both hold at multiples of chunk.
-bool graph_usable(const ArmonSolver* s, int64_t cycle)
-{
-    const int chunk = s->p.graph_chunk;
-    return chunk > 0 && chunk % 2 == 0 && cycle % chunk == 0;
-}
-
+// n more cycles: graph launches of graph_chunk cycles while at least that many remain, then
+// the rest one cycle at a time.
 void fused_advance(ArmonSolver* s, int64_t n)
 {
+    const int chunk = s->p.graph_chunk;
     while (n > 0) {
         const int64_t c = s->cycles_enqueued;
-        if (graph_usable(s, c) && n >= s->p.graph_chunk) {
-            if (!s->graph) fused_build_graph(s);
-            hip_check(hipGraphLaunch(s->graph, s->stream), "hipGraphLaunch");
-            s->cycles_enqueued += s->p.graph_chunk;
-            n -= s->p.graph_chunk;
+        if (chunk > 0 && n >= chunk) {
+            const int parity = (int) (c & 1);
+            if (!s->graph[parity]) fused_build_graph(s, parity);
+            Stage st(s->clock, "cycles_graph", "fused cycles (graph)");
+            hip_check(hipGraphLaunch(s->graph[parity], s->stream), "hipGraphLaunch");
+            s->cycles_enqueued += chunk;
+            n -= chunk;
         } else {
+            Stage st(s->clock, "cycle", "fused cycles (launch by launch)");
             fused_enqueue_cycle(s, c, nullptr);
             s->cycles_enqueued++;
             n--;
@@ -548,6 +563,10 @@ void fused_begin(ArmonSolver* s)
                             p.domain_size[1] / (double) p.ny, s->stream);
     hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     s->cycles_enqueued = 0;
+    // both graphs now, so that no later advance pays for a capture
+    if (p.graph_chunk > 0)
+        for (int parity = 0; parity < 2; parity++)
+            if (!s->graph[parity]) fused_build_graph(s, parity);
 }
 
 // After the loop: put the final state in rho/umat/vmat/Emat and the EOS of the last sweep's
@@ -631,8 +650,10 @@ void fill_result(ArmonSolver* s, ArmonResult* out, double loop_time, int status)
 
 ArmonSolver::~ArmonSolver()
 {
-    if (graph) (void) hipGraphExecDestroy(graph);
-    if (graph_def) (void) hipGraphDestroy(graph_def);
+    for (int k = 0; k < 2; k++) {
+        if (graph[k]) (void) hipGraphExecDestroy(graph[k]);
+        if (graph_def[k]) (void) hipGraphDestroy(graph_def[k]);
+    }
     for (auto* a : arr)
         if (a) (void) hipFree(a);
     for (auto* a : f32)
@@ -758,6 +779,13 @@ ARMON_EXPORT int armon_params_check(const ArmonParams* p)
     }
     if (p->compare && p->output_file == nullptr) {
         fprintf(stderr, "ERROR: the per-step comparison needs an output file name (the step files' prefix)\n");
+        return 0;
+    }
+    if (p->write_output && p->write_ghosts && p->engine == ARMON_ENGINE_FUSED) {
+        // the fused sweeps synthesise ghost cells in registers and LDS and never store them, so
+        // the arrays' ghost layers keep their initial values (the reference writes them at every
+        // boundary_conditions call, halo_exchange.cpp:23-30)
+        fprintf(stderr, "ERROR: writing the ghost cells needs the staged engine (the fused engine keeps none)\n");
         return 0;
     }
     if (p->engine == ARMON_ENGINE_STAGED && p->precision != ARMON_PREC_FP64) {
@@ -950,6 +978,8 @@ ARMON_EXPORT ArmonSolver* armon_solver_create(const ArmonParams* params)
         s->n = s->p.nb_cells;
         const size_t bytes = sizeof(double) * (size_t) s->n;
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+        s->clock.set_stream(s->stream);
+        s->clock.enable(s->p.verbose < 3);
         for (int f = 0; f < ARMON_FIELD_COUNT; f++) {
             void* ptr = nullptr;
             hip_check(hipMalloc(&ptr, bytes), "hipMalloc(field)");
@@ -1004,6 +1034,7 @@ ARMON_EXPORT int armon_solver_init_test(ArmonSolver* s, bool debug_indexes)
         DomainRange d = armon_complete_domain(&p);
         auto [range, inner] = iter2D(d);
         const double test_option = armon_test_option(&p);
+        Stage st(s->clock, "init_test", "init_test");
         init_test(range, inner, p.row_length, p.nb_ghosts, p.nx, p.ny, p.domain_size[0], p.domain_size[1],
                   p.domain_origin[0], p.domain_origin[1], 0, 0, p.nx, p.ny, V(s, ARMON_FIELD_X), V(s, ARMON_FIELD_Y),
                   V(s, ARMON_FIELD_RHO), V(s, ARMON_FIELD_EMAT), V(s, ARMON_FIELD_UMAT), V(s, ARMON_FIELD_VMAT),
@@ -1033,7 +1064,7 @@ ARMON_EXPORT int armon_solver_begin(ArmonSolver* s)
         s->t_begin = std::chrono::steady_clock::now();
         s->compare_stop = false;
         armon_update_axis(&p, ARMON_AXIS_X);
-        staged_update_EOS(s);
+        { Stage st(s->clock, "EOS_init", nullptr); staged_update_EOS(s); }
         step_checkpoint(s, "EOS_init");
         if (p.verbose <= 1) conservation(s, &s->initial_mass, &s->initial_energy);
         if (p.engine == ARMON_ENGINE_FUSED) fused_begin(s);
@@ -1059,6 +1090,7 @@ ARMON_EXPORT int armon_solver_advance(ArmonSolver* s, int n)
         for (int k = 0; k < n && p.time < p.max_time && p.cycle < p.max_cycles; k++) {
             const int e0 = g_error_count;
             if (!staged_solver_cycle(s) || g_error_count != e0) return 1;
+            s->clock.collect(false);   // the stages that have completed (dt_CFL waited for the cycle's start)
             if (p.verbose <= 1) {
                 double m, e;
                 conservation(s, &m, &e);
@@ -1112,6 +1144,7 @@ ARMON_EXPORT int armon_solver_finish(ArmonSolver* s, ArmonResult* out)
         }
         const double loop_time =
                 std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_begin).count();
+        s->clock.collect(true);
         if (g_error_count != s->error_count_at_begin && status == 0) status = 2;
         fill_result(s, out, loop_time, status);
         if (out && p.verbose <= 1) conservation(s, &out->final_mass, &out->final_energy);
@@ -1161,6 +1194,23 @@ ARMON_EXPORT int armon_solver_time_loop(ArmonSolver* s, ArmonResult* out)
         }
     }
     return armon_solver_finish(s, out);
+}
+
+ARMON_EXPORT int armon_solver_stage_count(ArmonSolver* s)
+{
+    s->clock.collect(true);
+    return (int) s->clock.table().size();
+}
+
+ARMON_EXPORT int armon_solver_stage_time(ArmonSolver* s, int k, const char** label, double* ms, int64_t* calls)
+{
+    if (k < 0 || k >= (int) s->clock.table().size()) return 1;
+    auto it = s->clock.table().begin();
+    std::advance(it, k);
+    if (label) *label = it->first.c_str();
+    if (ms) *ms = it->second.ms;
+    if (calls) *calls = it->second.calls;
+    return 0;
 }
 
 ARMON_EXPORT int armon_solver_kernel_times(ArmonSolver* s, int n, ArmonKernelTimes* out)
@@ -1427,6 +1477,14 @@ extern "C" ARMON_EXPORT int armon_run(ArmonParams* p, ArmonResult* out)
         printf("Cells/sec:  %.4g Mega cells/sec\n", out->grind_time > 0 ? 1. / out->grind_time : 0.0);
         printf("Cycles:     %d\n", out->cycles);
         printf("Final dt:   %.18f\n\n", out->final_dt);
+    }
+    if (status == 0 && p->verbose < 3) {
+        // armon_2D.cpp:420-428, with device time per stage
+        double total = 0;
+        for (const auto& [label, e] : s->clock.table()) total += e.ms;
+        printf("Total time for each step:\n");
+        for (const auto& [label, e] : s->clock.table())
+            printf(" - %-32s %10.5f ms (%5.2f%%)\n", label.c_str(), e.ms, total > 0 ? e.ms / total * 100 : 0.0);
     }
     if (status == 0 && p->write_output) write_output(s, p->output_file);
     if (status == 0 && p->write_throughput) {

@@ -113,11 +113,11 @@ __global__ void __launch_bounds__(256) k_eos_real(const T* rho, const T* u, cons
 }  // namespace
 
 template void launch_sweep<double>(int, const SweepVariant&, const SweepArgs<double>&, hipStream_t);
-template void launch_batch_sweep<double>(int, const SweepVariant&, const BatchArgs<double>&, const void*, int, int,
-                                         hipStream_t);
-template int batch_resident_blocks<double>(int, const SweepVariant&);
+template void launch_batch_sweep<double>(int, const SweepVariant&, const BatchArgs<double>&, const void*, int, int, int,
+                                       hipStream_t);
+template int batch_resident_blocks<double>(int, const SweepVariant&, int);
 
-int x_tile_outputs(const SweepVariant& v) { return kTile - 2 * read_extent(v.flux, v.proj); }
+int x_tile_outputs(const SweepVariant& v, int tile) { return tile - 2 * read_extent(v.flux, v.proj); }
 
 void launch_time_step(DeviceState* st, const TimeStepConfig& c, hipStream_t s)
 {

@@ -37,7 +37,7 @@ struct SweepArgs {
     int seg_len;                          // Y sweep: rows per marching run (set by the launcher)
     int x_slots;                          // X sweep: slots per row, nx + 2 x the ghost layers read
     float x_inv_slots;                    // 1 / x_slots
-    int dx_pow2, reduce;
+    int dx_mode, reduce;                  // dx_mode: DX_POW2 (inv_dx = 1 / dx) or DX_RECIP (inv_dx = RN(1 / dx))
     double gamma, dx, inv_dx, dt_factor;
     double lo_u, lo_v, hi_u, hi_v;        // BC factors: Left/Bottom (lo), Right/Top (hi)
     double cfl_dx, cfl_dy;
@@ -56,8 +56,8 @@ inline int read_extent(int flux, int proj) { return 2 + (flux > 0 ? 1 : 0) + (pr
 template <class T>
 void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStream_t s);
 
-// Output cells per X-sweep tile.
-int x_tile_outputs(const SweepVariant& v);
+// Output cells per X-sweep tile of `tile` threads (the single-domain sweep's: 256).
+int x_tile_outputs(const SweepVariant& v, int tile = 256);
 
 void launch_time_step(DeviceState* st, const TimeStepConfig& c, hipStream_t s);
 
@@ -67,8 +67,7 @@ void launch_time_step(DeviceState* st, const TimeStepConfig& c, hipStream_t s);
 struct BatchDomain {
     long long offset;              // first element of the domain in the packed arrays
     int nx, ny, row_length, index_start;
-    int dx_pow2[2];                // per axis
-    double dx[2], inv_dx[2];       // per axis
+    double dx[2], inv_dx[2];       // per axis; inv_dx = RN(1 / dx), for the batch's DX_RECIP division
     double cfl_dx, cfl_dy;
     double bc[4][2];               // per side (Left, Right, Top, Bottom): u, v factors
     int x_slots;                   // X sweep: slots per row (nx + 2 x the ghost layers read)
@@ -92,14 +91,16 @@ struct BatchArgs {
 };
 
 // One launch sweeps every domain of the batch along `axis`: `grid` persistent blocks walk
-// the work items (BatchItemX for X, BatchItemY for Y).
+// the work items (BatchItemX for X, BatchItemY for Y); X tiles are `tile` threads wide (128,
+// 256 or 512, the width the X items were cut for).
 template <class T>
 void launch_batch_sweep(int axis, const SweepVariant& v, const BatchArgs<T>& b, const void* items, int nitems,
-                        int grid, hipStream_t s);
+                        int grid, int tile, hipStream_t s);
 
-// Blocks of the batched sweep kernel of this variant that fit on the device at once.
+// Blocks of the batched sweep kernel of this variant (X tiles of `tile` threads) that fit on the
+// device at once.
 template <class T>
-int batch_resident_blocks(int axis, const SweepVariant& v);
+int batch_resident_blocks(int axis, const SweepVariant& v, int tile);
 
 // The time step of every domain (one thread each), per-domain configuration.
 void launch_batch_time_step(DeviceState* states, const TimeStepConfig* cfgs, int n, hipStream_t s);

@@ -15,8 +15,8 @@
 //    window), so no LDS and no barriers; consecutive lanes touch consecutive columns.
 //  * Both number their blocks XCD by XCD (xcd_block) so that each XCD's L2 serves a
 //    contiguous part of the arrays.
-//  * The batched ensemble (k_batch_x, k_batch_x1, k_batch_y) runs the same tile and run
-//    bodies over a work list that spans many domains.
+//  * The batched ensemble (k_batch_x, k_batch_x1, k_batch_y, k_batch_y1) runs the same tile and
+//    run bodies over a work list that spans many domains.
 //
 // Ghost cells are synthesised in registers: layers within the stencil width mirror the real
 // cells with the side's velocity factors (halo_exchange.cpp:9-36); deeper layers keep their
@@ -41,7 +41,7 @@
 namespace armon {
 namespace sweep {
 
-constexpr int kTile = 256;    // X sweep: threads (cells) per tile
+constexpr int kTile = 256;    // X sweep: threads (cells) per tile of the single-domain sweep
 constexpr int kYBlock = 256;  // Y sweep: columns per block
 
 // Y march: rows per unrolled loop body, the minimum waves per SIMD the register allocation
@@ -94,8 +94,9 @@ __device__ __forceinline__ void cell_eos(double gamma, T rho, T u, T v, T E, T& 
 }
 
 // Cell `k` (along the sweep axis, `n` real cells) of pencil `q` (row for X, column for Y).
-// Real cells are loaded; ghost layers 1..stencil_width mirror real cells with the side's
-// factors; deeper ghost layers are read as they are, with p and c from stale_p/stale_c.
+// Real cells are loaded; ghost layers 1..stencil_width mirror real cells: the real cell's
+// rho, E, p and c, its velocities times the side's factors; deeper ghost layers are read as
+// they are, with p and c from stale_p/stale_c.
 template <class T, int AXIS, int EOS>
 __device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k, int n)
 {
@@ -111,6 +112,15 @@ __device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k
     }
     const int idx = AXIS == 0 ? a.index_start + q * a.row_length + kk : a.index_start + kk * a.row_length + q;
     T rho = a.in_rho[idx], u = a.in_u[idx], v = a.in_v[idx], E = a.in_E[idx];
+    Cell<T> cell;
+    if (stale) {
+        cell.p = a.stale_p[idx];
+        cell.c = a.stale_c[idx];
+    } else {
+        // a mirror ghost takes the p and c of its real cell (halo_exchange.cpp:23-30): the EOS
+        // of the unscaled state
+        cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);
+    }
     if (mirror == 1) {
         u = u * (T) a.lo_u;
         v = v * (T) a.lo_v;
@@ -118,17 +128,10 @@ __device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k
         u = u * (T) a.hi_u;
         v = v * (T) a.hi_v;
     }
-    Cell<T> cell;
     cell.rho = rho;
     cell.E = E;
     cell.un = AXIS == 0 ? u : v;
     cell.ut = AXIS == 0 ? v : u;
-    if (stale) {
-        cell.p = a.stale_p[idx];
-        cell.c = a.stale_c[idx];
-    } else {
-        cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);
-    }
     return cell;
 }
 
@@ -161,11 +164,18 @@ __device__ __forceinline__ CellSource cell_source(const SweepArgs<T>& a, int q, 
     return s;
 }
 
-// The cell from its loaded state (rho, u, v, E at s.idx): mirror factors, then EOS (or the
-// stale p and c).
+// The cell from its loaded state (rho, u, v, E at s.idx): EOS (or the stale p and c), then the
+// mirror factors.
 template <class T, int AXIS, int EOS, bool STALE = true>
 __device__ __forceinline__ Cell<T> make_cell(const SweepArgs<T>& a, const CellSource& s, T rho, T u, T v, T E)
 {
+    Cell<T> cell;
+    if (STALE && s.stale) {
+        cell.p = a.stale_p[s.idx];
+        cell.c = a.stale_c[s.idx];
+    } else {
+        cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);   // unscaled, as load_cell
+    }
     if (s.mirror == 1) {
         u = u * (T) a.lo_u;
         v = v * (T) a.lo_v;
@@ -173,17 +183,10 @@ __device__ __forceinline__ Cell<T> make_cell(const SweepArgs<T>& a, const CellSo
         u = u * (T) a.hi_u;
         v = v * (T) a.hi_v;
     }
-    Cell<T> cell;
     cell.rho = rho;
     cell.E = E;
     cell.un = AXIS == 0 ? u : v;
     cell.ut = AXIS == 0 ? v : u;
-    if (STALE && s.stale) {
-        cell.p = a.stale_p[s.idx];
-        cell.c = a.stale_c[s.idx];
-    } else {
-        cell_eos<T, EOS>(a.gamma, rho, u, v, E, cell.p, cell.c);
-    }
     return cell;
 }
 
@@ -224,42 +227,42 @@ __device__ __forceinline__ void reduce_cfl(DeviceState* st, double local)
 // p, c and the Godunov fluxes until the face fluxes are done (stages 0-2), then the advection
 // fluxes (stages 4-5); a barrier separates the last read of the former from the first write
 // of the latter. 10 arrays: 20 KiB in FP64, so that 8 blocks fit in a CU's LDS.
-template <class T>
+template <class T, int TILE = kTile>
 struct SmemX {
-    T rho[kTile], un[kTile], ut[kTile], E[kTile], us[kTile], ps[kTile];
-    T x[4][kTile];
+    T rho[TILE], un[TILE], ut[TILE], E[TILE], us[TILE], ps[TILE];
+    T x[4][TILE];
     __device__ T* p() { return x[0]; }
     __device__ T* c() { return x[1]; }
     __device__ T* gu() { return x[2]; }
     __device__ T* gp() { return x[3]; }
-    __device__ T (&f())[4][kTile] { return x; }
+    __device__ T (&f())[4][TILE] { return x; }
 };
 
 // One X tile of the domain described by `a`, with the cycle's dt. Called by all threads of a
-// kTile-thread block.
+// TILE-thread block.
 //
 // The rows of the domain are laid end to end as a stream of slots, each row taking
 // x_slots = nx + 2R slots (its R ghost cells on each side, then the next row). A tile covers
-// kTile consecutive slots, starting at slot `k0` of row `j0`; consecutive tiles overlap by 2R
-// slots, and the slots [R, kTile - R) of a tile that hold real cells are its outputs. A cell's
+// TILE consecutive slots, starting at slot `k0` of row `j0`; consecutive tiles overlap by 2R
+// slots, and the slots [R, TILE - R) of a tile that hold real cells are its outputs. A cell's
 // result depends only on the cells within R of it in its own row, so a tile may span the end
 // of one row and the start of the next; no thread is spent on a tile's share of a row end.
-template <class T, int FLUX, int PROJ, int EOS>
-__device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0, int k0, SmemX<T>& sm)
+template <class T, int FLUX, int PROJ, int EOS, int DXM, int TILE = kTile>
+__device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0, int k0, SmemX<T, TILE>& sm)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
     constexpr int E2 = PROJ == 2 ? 1 : 0;
     constexpr int R = 2 + GAD + E2;
-    constexpr int W = kTile - 2 * R;
+    constexpr int W = TILE - 2 * R;
     const T dx = (T) a.dx;
 
     const int t = threadIdx.x;
-    // row and cell of this thread's slot: n = k0 + t < x_slots + kTile. Rows of at least kTile
+    // row and cell of this thread's slot: n = k0 + t < x_slots + TILE. Rows of at least TILE
     // slots: n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and
     // a single correction makes it exact.
     const int n = k0 + t;
     int dj, r;
-    if (a.x_slots >= kTile) {
+    if (a.x_slots >= TILE) {
         dj = n >= a.x_slots ? 1 : 0;
         r = n - (dj ? a.x_slots : 0);
     } else {
@@ -272,7 +275,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
     const int k = r - R;                  // -R <= k < nx + R
     const bool in_range = j < a.ny;       // slots past the last row
     const int tm = t > 0 ? t - 1 : 0;
-    const int tp = t < kTile - 1 ? t + 1 : kTile - 1;
+    const int tp = t < TILE - 1 ? t + 1 : TILE - 1;
 
     // Stage 0: load (or synthesise) the cell, EOS.
     const Cell<T> cell = in_range ? load_cell<T, 0, EOS>(a, j, k, a.nx) : benign_cell<T>();
@@ -324,8 +327,8 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
             td = t;
         }
         const int tdm = td > 0 ? td - 1 : 0;
-        const int tdp = td < kTile - 1 ? td + 1 : kTile - 1;
-        const int tdpp = td < kTile - 2 ? td + 2 : kTile - 1;
+        const int tdp = td < TILE - 1 ? td + 1 : TILE - 1;
+        const int tdpp = td < TILE - 2 ? td + 2 : TILE - 1;
         advection_flux_2<T>(dx, dt, disp, Dx, sm.us[tdm], sm.us[td], sm.us[tdp], sm.us[tdpp], sm.rho[tdm], sm.rho[td],
                             sm.rho[tdp], sm.un[tdm], sm.un[td], sm.un[tdp], sm.ut[tdm], sm.ut[td], sm.ut[tdp], sm.E[tdm],
                             sm.E[td], sm.E[tdp], f0, f1, f2, f3);
@@ -342,13 +345,8 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
     // Stage 5: projection of cell k, output.
     double local = DBL_MAX;
     if (t >= R && t < R + W && in_range && k >= 0 && k < a.nx) {
-        const T inv_dx = (T) a.inv_dx;
-        if (a.dx_pow2)
-            projection<true, T>(dx, inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
-                                sm.f()[3][tp], rho, un, ut, E);
-        else
-            projection<false, T>(dx, inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
-                                 sm.f()[3][tp], rho, un, ut, E);
+        projection<DXM, T>(dx, (T) a.inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
+                           sm.f()[3][tp], rho, un, ut, E);
         const int idx = a.index_start + j * a.row_length + k;
         a.out_rho[idx] = rho;
         a.out_u[idx] = un;
@@ -359,7 +357,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
     if (a.reduce) reduce_cfl(a.st, local);
 }
 
-template <class T, int FLUX, int PROJ, int EOS>
+template <class T, int FLUX, int PROJ, int EOS, int DXM>
 __global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a)
 {
     constexpr int W = kTile - 2 * (2 + (FLUX > 0 ? 1 : 0) + (PROJ == 2 ? 1 : 0));
@@ -368,7 +366,7 @@ __global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a)
     __shared__ SmemX<T> sm;
     const int s0 = xcd_block(blockIdx.x, gridDim.x) * W;   // first slot of the tile
     const int j0 = s0 / a.x_slots;
-    sweep_x_tile<T, FLUX, PROJ, EOS>(a, dt, j0, s0 - j0 * a.x_slots, sm);
+    sweep_x_tile<T, FLUX, PROJ, EOS, DXM>(a, dt, j0, s0 - j0 * a.x_slots, sm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -388,7 +386,7 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
 // row's state one row before it is used (the FP64 sweeps); with STALE = false, when the
 // stencil width covers the read extent so that no ghost layer is stale, the march then loads
 // nothing else.
-template <class T, int FLUX, int PROJ, int EOS, bool STALE, bool AHEAD>
+template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD>
 __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, bool lane_ok, int y0, int y1)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
@@ -514,12 +512,8 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
             constexpr int QP = E2 ? 1 : 2;    // q index of cell P
             constexpr int FP = E2 ? 1 : 2;    // f_u index of face P
             T r = q_rho[QP], u = q_un[QP], v = q_ut[QP], e = q_E[QP];
-            if (a.dx_pow2)
-                projection<true, T>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1], h2[0], h2[1],
-                                    h3[0], h3[1], r, u, v, e);
-            else
-                projection<false, T>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1], h2[0],
-                                     h2[1], h3[0], h3[1], r, u, v, e);
+            projection<DXM, T>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1], h2[0], h2[1], h3[0],
+                               h3[1], r, u, v, e);
             const int idx = a.index_start + P * a.row_length + x;
             a.out_rho[idx] = r;
             a.out_u[idx] = v;   // ut is the x velocity in a Y sweep
@@ -532,7 +526,7 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
     if (a.reduce) reduce_cfl(a.st, local);
 }
 
-template <class T, int FLUX, int PROJ, int EOS, bool STALE>
+template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE>
 __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs<T> a, int ncolblk)
 {
     if (!a.st->active) return;
@@ -543,12 +537,18 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     if (x - (int) (threadIdx.x & 63) >= a.nx) return;   // a wave past the last column (no barriers follow)
     const int y0 = seg * a.seg_len;
     constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
-    sweep_y_run<T, FLUX, PROJ, EOS, STALE, ahead>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
+    sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
 }
 
 // ---------------------------------------------------------------------------------------
 // Batched ensemble: persistent blocks over the work items of every domain
 // ---------------------------------------------------------------------------------------
+
+// How the batched sweeps divide by the cell sizes, the same for every domain: the
+// reciprocal-corrected quotient in FP64 (exact for powers of two too), the FP32-mixed division
+// (a native reciprocal) in FP32.
+template <class T>
+constexpr int kBatchDxMode = sizeof(T) == 8 ? DX_RECIP : DX_DIVIDE;
 
 template <class T>
 __device__ __forceinline__ SweepArgs<T> batch_domain_args(const BatchArgs<T>& b, int d, int axis)
@@ -572,7 +572,6 @@ __device__ __forceinline__ SweepArgs<T> batch_domain_args(const BatchArgs<T>& b,
     a.index_start = D.index_start;
     a.stencil_width = b.stencil_width;
     a.seg_len = 0;
-    a.dx_pow2 = D.dx_pow2[axis];
     a.reduce = b.reduce;
     a.gamma = b.gamma;
     a.dx = D.dx[axis];
@@ -590,31 +589,33 @@ __device__ __forceinline__ SweepArgs<T> batch_domain_args(const BatchArgs<T>& b,
     return a;
 }
 
-template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kTile) k_batch_x(BatchArgs<T> b, const BatchItemX* __restrict__ items, int nitems)
+template <class T, int FLUX, int PROJ, int EOS, int TILE>
+__global__ void __launch_bounds__(TILE) k_batch_x(BatchArgs<T> b, const BatchItemX* __restrict__ items, int nitems)
 {
-    __shared__ SmemX<T> sm;
+    __shared__ SmemX<T, TILE> sm;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const BatchItemX item = items[it];
         const DeviceState* st = b.states + item.domain;
         if (!st->active) continue;   // the same for the whole block
         const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
-        sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.slot, sm);
+        sweep_x_tile<T, FLUX, PROJ, EOS, kBatchDxMode<T>, TILE>(a, (T) (st->current_dt * b.dt_factor), item.row,
+                                                               item.slot, sm);
         __syncthreads();             // the next item reuses the LDS
     }
 }
 
 // One block per X item (the launch configuration the autotuner usually picks): no item loop,
 // so the tile body keeps the single-domain kernel's register budget.
-template <class T, int FLUX, int PROJ, int EOS>
-__global__ void __launch_bounds__(kTile) k_batch_x1(BatchArgs<T> b, const BatchItemX* __restrict__ items)
+template <class T, int FLUX, int PROJ, int EOS, int TILE>
+__global__ void __launch_bounds__(TILE) k_batch_x1(BatchArgs<T> b, const BatchItemX* __restrict__ items)
 {
-    __shared__ SmemX<T> sm;
+    __shared__ SmemX<T, TILE> sm;
     const BatchItemX item = items[xcd_block(blockIdx.x, gridDim.x)];
     const DeviceState* st = b.states + item.domain;
     if (!st->active) return;
     const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
-    sweep_x_tile<T, FLUX, PROJ, EOS>(a, (T) (st->current_dt * b.dt_factor), item.row, item.slot, sm);
+    sweep_x_tile<T, FLUX, PROJ, EOS, kBatchDxMode<T>, TILE>(a, (T) (st->current_dt * b.dt_factor), item.row,
+                                                           item.slot, sm);
 }
 
 // The row-ahead loads of the FP64 march need 179 VGPRs here (the per-item domain arguments
@@ -636,9 +637,28 @@ __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)
         const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
         const int x = item.x0 + lane;
         constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
-        sweep_y_run<T, FLUX, PROJ, EOS, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x, x < a.nx, item.y0,
-                                                      item.y1);
+        sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
+                                                                       x < a.nx, item.y0, item.y1);
     }
+}
+
+// One wave per Y item (the launch configuration the autotuner usually picks): no item loop, so
+// the run body keeps the single-domain kernel's register budget.
+template <class T, int FLUX, int PROJ, int EOS, bool STALE>
+__global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)))
+        k_batch_y1(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
+{
+    const int lane = threadIdx.x & 63;
+    const int it = __builtin_amdgcn_readfirstlane((int) ((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (it >= nitems) return;   // the whole wave
+    const BatchItemY item = items[it];
+    const DeviceState* st = b.states + item.domain;
+    if (!st->active) return;
+    const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
+    const int x = item.x0 + lane;
+    constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
+    sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
+                                                                   x < a.nx, item.y0, item.y1);
 }
 
 // X tiles covering a stream of `slots` slots (tiles of W outputs overlapping by 2R): the last
@@ -714,7 +734,7 @@ inline void check_launch(const char* what)
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-template <class T, int F, int P, int E>
+template <class T, int F, int P, int E, int DXM>
 void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
 {
     constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
@@ -724,7 +744,7 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
         b.x_slots = a.nx + 2 * R;
         b.x_inv_slots = 1.0f / (float) b.x_slots;
         const int64_t tiles = x_tiles((int64_t) a.ny * b.x_slots, R, W);
-        k_sweep_x<T, F, P, E><<<dim3((unsigned) tiles), dim3(kTile), 0, s>>>(b);
+        k_sweep_x<T, F, P, E, DXM><<<dim3((unsigned) tiles), dim3(kTile), 0, s>>>(b);
         check_launch("sweep_x");
     } else {
         // Each block marches down a run of rows of its 256 columns (the warm-up is paid once
@@ -735,18 +755,28 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
         const int nseg = (a.ny + b.seg_len - 1) / b.seg_len;
         const dim3 grid((unsigned) (ncolblk * nseg));
         if (a.stencil_width >= R)
-            k_sweep_y<T, F, P, E, false><<<grid, dim3(kYBlock), 0, s>>>(b, ncolblk);
+            k_sweep_y<T, F, P, E, DXM, false><<<grid, dim3(kYBlock), 0, s>>>(b, ncolblk);
         else
-            k_sweep_y<T, F, P, E, true><<<grid, dim3(kYBlock), 0, s>>>(b, ncolblk);
+            k_sweep_y<T, F, P, E, DXM, true><<<grid, dim3(kYBlock), 0, s>>>(b, ncolblk);
         check_launch("sweep_y");
     }
+}
+
+// The division by dx of the single-domain sweeps: a multiply for a power of two, else the
+// reciprocal-corrected quotient in FP64 and the FP32-mixed division in FP32.
+template <class T, int F, int P, int E>
+void launch_dx_dispatch(int axis, const SweepArgs<T>& a, hipStream_t s)
+{
+    constexpr int other = sizeof(T) == 8 ? DX_RECIP : DX_DIVIDE;
+    if (a.dx_mode == DX_POW2) launch_variant<T, F, P, E, DX_POW2>(axis, a, s);
+    else launch_variant<T, F, P, E, other>(axis, a, s);
 }
 
 template <class T, int F, int P>
 void launch_eos_dispatch(int axis, int eos, const SweepArgs<T>& a, hipStream_t s)
 {
-    if (eos == 0) launch_variant<T, F, P, 0>(axis, a, s);
-    else launch_variant<T, F, P, 1>(axis, a, s);
+    if (eos == 0) launch_dx_dispatch<T, F, P, 0>(axis, a, s);
+    else launch_dx_dispatch<T, F, P, 1>(axis, a, s);
 }
 
 template <class T, int F>
@@ -756,32 +786,55 @@ void launch_proj_dispatch(int axis, const SweepVariant& v, const SweepArgs<T>& a
     else launch_eos_dispatch<T, F, 1>(axis, v.eos, a, s);
 }
 
+template <class T, int F, int P, int E, int TILE>
+void launch_batch_x(const BatchArgs<T>& b, const BatchItemX* items, int nitems, int grid, hipStream_t s)
+{
+    if (grid >= nitems)
+        k_batch_x1<T, F, P, E, TILE><<<dim3((unsigned) nitems), dim3(TILE), 0, s>>>(b, items);
+    else
+        k_batch_x<T, F, P, E, TILE><<<dim3((unsigned) grid), dim3(TILE), 0, s>>>(b, items, nitems);
+}
+
 template <class T, int F, int P, int E>
-void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, int nitems, int grid, hipStream_t s)
+void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, int nitems, int grid, int tile,
+                          hipStream_t s)
 {
     if (nitems <= 0) return;
-    if (axis == 0 && grid >= nitems)
-        k_batch_x1<T, F, P, E><<<dim3((unsigned) nitems), dim3(kTile), 0, s>>>(b, static_cast<const BatchItemX*>(items));
-    else if (axis == 0)
-        k_batch_x<T, F, P, E><<<dim3((unsigned) grid), dim3(kTile), 0, s>>>(b, static_cast<const BatchItemX*>(items),
-                                                                             nitems);
-    else
-    {
+    if (axis == 0) {
+        const auto* x_items = static_cast<const BatchItemX*>(items);
+        switch (tile) {
+        case 128: launch_batch_x<T, F, P, E, 128>(b, x_items, nitems, grid, s); break;
+        case 512: launch_batch_x<T, F, P, E, 512>(b, x_items, nitems, grid, s); break;
+        default: launch_batch_x<T, F, P, E, 256>(b, x_items, nitems, grid, s); break;
+        }
+    } else {
         constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
         const auto* y_items = static_cast<const BatchItemY*>(items);
-        if (b.stencil_width >= R)
+        const int waves_per_block = kYBlock / 64;
+        if ((int64_t) grid * waves_per_block >= nitems) {
+            const unsigned g1 = (unsigned) ((nitems + waves_per_block - 1) / waves_per_block);
+            if (b.stencil_width >= R)
+                k_batch_y1<T, F, P, E, false><<<dim3(g1), dim3(kYBlock), 0, s>>>(b, y_items, nitems);
+            else
+                k_batch_y1<T, F, P, E, true><<<dim3(g1), dim3(kYBlock), 0, s>>>(b, y_items, nitems);
+        } else if (b.stencil_width >= R) {
             k_batch_y<T, F, P, E, false><<<dim3((unsigned) grid), dim3(kYBlock), 0, s>>>(b, y_items, nitems);
-        else
+        } else {
             k_batch_y<T, F, P, E, true><<<dim3((unsigned) grid), dim3(kYBlock), 0, s>>>(b, y_items, nitems);
+        }
     }
     check_launch(axis == 0 ? "batch_x" : "batch_y");
 }
 
 template <class T, int F, int P, int E>
-int batch_variant_resident(int axis)
+int batch_variant_resident(int axis, int tile)
 {
-    return axis == 0 ? resident_blocks((const void*) k_batch_x<T, F, P, E>, kTile)
-                     : resident_blocks((const void*) k_batch_y<T, F, P, E, false>, kYBlock);
+    if (axis != 0) return resident_blocks((const void*) k_batch_y<T, F, P, E, false>, kYBlock);
+    switch (tile) {
+    case 128: return resident_blocks((const void*) k_batch_x<T, F, P, E, 128>, 128);
+    case 512: return resident_blocks((const void*) k_batch_x<T, F, P, E, 512>, 512);
+    default: return resident_blocks((const void*) k_batch_x<T, F, P, E, 256>, 256);
+    }
 }
 
 // Calls fn.template operator()<F, P, E>() for the variant's compile-time parameters.
@@ -818,18 +871,19 @@ void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStr
 
 template <class T>
 void launch_batch_sweep(int axis, const SweepVariant& v, const BatchArgs<T>& b, const void* items, int nitems, int grid,
-                        hipStream_t s)
+                        int tile, hipStream_t s)
 {
     sweep::with_variant(v, [&]<int F, int P, int E>() {
-        sweep::launch_batch_variant<T, F, P, E>(axis, b, items, nitems, grid, s);
+        sweep::launch_batch_variant<T, F, P, E>(axis, b, items, nitems, grid, tile, s);
         return 0;
     });
 }
 
 template <class T>
-int batch_resident_blocks(int axis, const SweepVariant& v)
+int batch_resident_blocks(int axis, const SweepVariant& v, int tile)
 {
-    return sweep::with_variant(v, [&]<int F, int P, int E>() { return sweep::batch_variant_resident<T, F, P, E>(axis); });
+    return sweep::with_variant(v,
+                               [&]<int F, int P, int E>() { return sweep::batch_variant_resident<T, F, P, E>(axis, tile); });
 }
 
 }  // namespace armon

@@ -135,7 +135,10 @@ typedef struct ArmonParams {
     /* execution options of this implementation */
     int engine;                     /* ARMON_ENGINE_* */
     int precision;                  /* ARMON_PREC_* (fused engine only) */
-    int graph_chunk;                /* cycles per captured graph launch (fused engine); 0: no graph */
+    int graph_chunk;                /* cycles per captured graph launch (fused engine); 0: no graph.
+                                       Advancing n cycles launches n / graph_chunk graphs (one graph
+                                       per parity of the start cycle, captured at begin), then the
+                                       remaining n % graph_chunk cycles launch by launch. */
     /* per-step comparison with reference files (parameters.h:95-97; staged engine only) */
     int compare;
     double comparison_tolerance;    /* relative, default 1e-13 */
@@ -225,9 +228,21 @@ ARMON_EXPORT int armon_solver_advance(ArmonSolver* s, int n);
 ARMON_EXPORT int armon_solver_finish(ArmonSolver* s, ArmonResult* out);
 /* Runs n cycles launch by launch, timing every kernel with HIP events on the solver stream. */
 ARMON_EXPORT int armon_solver_kernel_times(ArmonSolver* s, int n, ArmonKernelTimes* out);
+/* Per-stage device time since the solver was created, under the reference's stage labels
+ * (armon_2D.cpp:57-67, 306-322: init_test, dt_CFL, EOS, boundary_conditions, numerical_fluxes,
+ * cell_update, euler_proj for the staged engine; the graph launches and eagerly launched cycles
+ * of the fused engine), each stage bracketed by HIP events on the solver stream. Recorded when
+ * the parameters' verbose < 3, when armon_run prints the table as the reference does. Stages are
+ * numbered 0 .. armon_solver_stage_count() - 1 in label order; label strings live as long as the
+ * solver. */
+ARMON_EXPORT int armon_solver_stage_count(ArmonSolver* s);
+ARMON_EXPORT int armon_solver_stage_time(ArmonSolver* s, int k, const char** label, double* ms, int64_t* calls);
 /* Copies a whole field (nb_cells values) to/from the host as doubles. For the fused
  * engine, fields are the current state; PMAT/CMAT hold the EOS of the input of the
- * last sweep, exactly as in the reference. */
+ * last sweep, exactly as in the reference. The fused engine keeps the real cells only: its
+ * sweeps make the ghost cells in registers and never store them, so ghost cells read back
+ * hold what init_test (or the last write_field) left there; the staged engine's ghost cells
+ * are the reference's. */
 ARMON_EXPORT int armon_solver_read_field(ArmonSolver* s, int field, double* host, int64_t n);
 ARMON_EXPORT int armon_solver_write_field(ArmonSolver* s, int field, const double* host, int64_t n);
 ARMON_EXPORT int armon_solver_conservation(ArmonSolver* s, double* mass, double* energy);
@@ -260,12 +275,16 @@ typedef struct ArmonDomainSpec {
 ARMON_EXPORT int armon_batch_random_specs(uint64_t seed, int n, int64_t min_area, int64_t max_area,
                                           double max_aspect, ArmonDomainSpec* out);
 
-/* Launch configuration of the batched sweeps, chosen by armon_batch_autotune. */
+/* Launch configuration of the batched sweeps, chosen by armon_batch_autotune: the tile shape
+ * and the cells each thread computes (an X tile is tile_x threads of one cell each along a row;
+ * a Y item is a wave of 64 columns, each thread marching y_run_rows cells down its column) and
+ * the launch grids. */
 typedef struct ArmonBatchTuning {
     int y_run_rows;           /* rows a wave marches per Y work item (64 columns wide) */
     int grid_x, grid_y;       /* blocks of the X and Y launches (persistent when below the item count) */
     int items_x, items_y;     /* work items per X / Y sweep (outputs) */
     double ms_x, ms_y;        /* measured time per X / Y sweep of this configuration (autotune output) */
+    int tile_x;               /* threads (cells) per X tile: 128, 256 or 512 (0 in set_tuning: 256) */
 } ArmonBatchTuning;
 
 typedef struct ArmonBatch ArmonBatch;
@@ -282,8 +301,9 @@ ARMON_EXPORT int armon_batch_domain_params(const ArmonBatch* b, int d, ArmonPara
 ARMON_EXPORT void* armon_batch_stream(ArmonBatch* b);
 /* init_test_states of every domain (armon_2D.cpp:112-127). */
 ARMON_EXPORT int armon_batch_init(ArmonBatch* b);
-/* Times the sweeps over the initial state for the candidate configurations (Y run length,
- * persistent or one-block-per-item grids) and keeps the fastest. After init, before begin. */
+/* Times the sweeps over the initial state for the candidate configurations (X tile width, Y
+ * run length, persistent or one-block-per-item grids) and keeps the fastest. After init, before
+ * begin. */
 ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out);
 /* Sets the configuration by hand (grid_x/grid_y <= 0: one block per item); before begin. */
 ARMON_EXPORT int armon_batch_set_tuning(ArmonBatch* b, const ArmonBatchTuning* t);

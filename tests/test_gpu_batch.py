@@ -86,9 +86,11 @@ def test_batch_matches_oracle_per_domain(armon, po, oracle):
     b.close()
 
 
-@pytest.mark.parametrize("tuning", [(16, 0, 0), (64, 7, 5), (1000, 1, 1), (33, 300, 0)])
+@pytest.mark.parametrize("tuning", [(16, 0, 0, 128), (64, 7, 5, 512), (1000, 1, 1, 256), (33, 300, 0, 128),
+                                    (8, 3, 0, 512)])
 def test_batch_launch_configurations(armon, po, oracle, tuning):
-    """Run lengths and persistent grids change the work split, never the result."""
+    """Y run lengths, X tile widths and persistent grids change the work split, never the result
+    (tuning = (Y run rows, X grid, Y grid, X tile width))."""
     specs = small_specs(armon, 5, 6)
     print(f"case batch seed 5 tuning {tuning}", flush=True)
     b = run_batch(armon, specs, tuning=tuning, **dict(BASE, max_cycles=25))
@@ -102,6 +104,7 @@ def test_batch_autotune(armon, po, oracle):
     b = run_batch(armon, specs, autotune=True, **dict(BASE, max_cycles=20))
     t = b.tuning()
     assert t.y_run_rows in (16, 32, 64, 128, 256, 512, 1024) and t.ms_x > 0 and t.ms_y > 0
+    assert t.tile_x in (128, 256, 512)
     assert t.items_x > 0 and t.items_y > 0
     compare_batch(armon, po, oracle, b, specs)
     b.close()
@@ -179,7 +182,8 @@ def test_batch_fp32_mixed_independent_of_launch_configuration(armon):
     same expressions, contracted the same way, wherever its tile or run falls)."""
     specs = small_specs(armon, 23, 6)
     ref = None
-    for tuning in [(16, 0, 0), (64, 7, 5), (1000, 1, 1), (33, 300, 0)]:
+    for tuning in [(16, 0, 0, 128), (64, 7, 5, 512), (1000, 1, 1, 256), (33, 300, 0, 128),
+                   (8, 3, 0, 512)]:
         print(f"case batch fp32 mixed seed 23 tuning {tuning}", flush=True)
         b = run_batch(armon, specs, tuning=tuning, **dict(BASE, max_cycles=25, precision=1))
         out = [b.read(k, f) for k in range(b.n) for f in STATE]
@@ -203,5 +207,17 @@ def test_batch_boundary_factor_override(armon, po, oracle):
     for k in range(b.n):
         bp = b.params(k)
         assert bp.bc_override == 1 and [tuple(bp.bc_factors[s]) for s in range(4)] == list(walls)
+    compare_batch(armon, po, oracle, b, specs)
+    b.close()
+
+
+def test_batch_non_unit_boundary_factors(armon, po, oracle):
+    """Boundary factors other than +-1 in the batch's common parameters (a no-slip wall and
+    partial factors): every domain bit for bit against the oracle with the same override, whose
+    mirror ghosts copy the real cells' p and c."""
+    specs = small_specs(armon, 37, 6)
+    print("case batch seed 37 n 6 non-unit factors", flush=True)
+    factors = ((0.0, 1.0), (0.5, 0.75), (1.0, 0.0), (-0.25, -1.0))
+    b = run_batch(armon, specs, bc_factors=factors, **BASE)
     compare_batch(armon, po, oracle, b, specs)
     b.close()
