@@ -107,7 +107,8 @@ class ArmonDomainSpec(C.Structure):
 
 class ArmonBatchTuning(C.Structure):
     _fields_ = [("y_run_rows", C.c_int), ("grid_x", C.c_int), ("grid_y", C.c_int), ("items_x", C.c_int),
-                ("items_y", C.c_int), ("ms_x", C.c_double), ("ms_y", C.c_double), ("tile_x", C.c_int)]
+                ("items_y", C.c_int), ("ms_x", C.c_double), ("ms_y", C.c_double), ("tile_x", C.c_int),
+                ("x_cells_per_thread", C.c_int)]
 
 
 class ArmonKernelTimes(C.Structure):
@@ -489,8 +490,10 @@ class Batch:
         load().armon_batch_get_tuning(self.h, C.byref(t))
         return t
 
-    def set_tuning(self, y_run_rows: int, grid_x: int = 0, grid_y: int = 0, tile_x: int = 0):
-        t = ArmonBatchTuning(y_run_rows=y_run_rows, grid_x=grid_x, grid_y=grid_y, tile_x=tile_x)
+    def set_tuning(self, y_run_rows: int, grid_x: int = 0, grid_y: int = 0, tile_x: int = 0,
+                   x_cells_per_thread: int = 0):
+        t = ArmonBatchTuning(y_run_rows=y_run_rows, grid_x=grid_x, grid_y=grid_y, tile_x=tile_x,
+                             x_cells_per_thread=x_cells_per_thread)
         _check(load().armon_batch_set_tuning(self.h, C.byref(t)), "armon_batch_set_tuning")
 
     def begin(self):

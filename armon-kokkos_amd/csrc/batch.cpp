@@ -233,10 +233,10 @@ void launch_one_sweep(ArmonBatch* b, int axis, int parity, double factor, bool r
     const int grid = x ? blocks_x(b, b->tuning.grid_x) : blocks_y(b, b->tuning.grid_y);
     if (b->fp64())
         launch_batch_sweep<double>(axis, b->var, batch_args<double>(b, parity, factor, reduce, states), items, nitems,
-                                   grid, b->tuning.tile_x, b->stream);
+                                   grid, b->tuning.tile_x, b->tuning.x_cells_per_thread, b->stream);
     else
         launch_batch_sweep<float>(axis, b->var, batch_args<float>(b, parity, factor, reduce, states), items, nitems,
-                                  grid, b->tuning.tile_x, b->stream);
+                                  grid, b->tuning.tile_x, b->tuning.x_cells_per_thread, b->stream);
 }
 
 struct Timer {
@@ -520,8 +520,10 @@ ARMON_EXPORT ArmonBatch* armon_batch_create(const ArmonParams* common, int n, co
         b->d_tune = device_alloc<DeviceState>(n, "hipMalloc(tune states)");
         hip_check(hipMemcpy(b->d_domains, dom.data(), sizeof(BatchDomain) * n, hipMemcpyHostToDevice), "upload domains");
         hip_check(hipMemcpy(b->d_cfgs, cfg.data(), sizeof(TimeStepConfig) * n, hipMemcpyHostToDevice), "upload configs");
-        // default configuration: 256-thread X tiles, 256-row Y runs, one block per work item
+        // default configuration: 256-cell X tiles of one cell per thread, 256-row Y runs, one
+        // block per work item
         b->tuning.tile_x = 256;
+        b->tuning.x_cells_per_thread = 1;
         b->tuning.grid_x = 0;
         b->tuning.grid_y = 0;
         build_items(b, 256);
@@ -592,7 +594,14 @@ ARMON_EXPORT int armon_batch_set_tuning(ArmonBatch* b, const ArmonBatchTuning* t
         if (t->y_run_rows < 1) throw invalid_argument("y_run_rows must be positive");
         if (t->tile_x != 0 && t->tile_x != 128 && t->tile_x != 256 && t->tile_x != 512)
             throw invalid_argument("tile_x must be 128, 256 or 512 (0: 256)");
-        b->tuning.tile_x = t->tile_x ? t->tile_x : 256;
+        const int tile = t->tile_x ? t->tile_x : 256;
+        const int cpt = t->x_cells_per_thread ? t->x_cells_per_thread : 1;
+        if (cpt != 1 && cpt != 2) throw invalid_argument("x_cells_per_thread must be 1 or 2 (0: 1)");
+        if (cpt == 2 && (tile == 128 || t->grid_x > 0))
+            throw invalid_argument("2 cells per thread needs X tiles of 256 or 512 cells and one block per tile "
+                                   "(grid_x <= 0)");
+        b->tuning.tile_x = tile;
+        b->tuning.x_cells_per_thread = cpt;
         b->tuning.grid_x = std::max(0, t->grid_x);
         b->tuning.grid_y = std::max(0, t->grid_y);
         build_items(b, t->y_run_rows);
@@ -624,11 +633,13 @@ ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out)
                                  : batch_resident_blocks<float>(ARMON_AXIS_Y, b->var, 256);
         const int reps = 5;
 
-        // X: tile width (128, 256, 512 threads, one cell each) x (persistent grids of 1, 2, 4
-        // resident rounds | one block per tile)
+        // X: tile width (128, 256, 512 cells, one per thread) x (persistent grids of 1, 2, 4
+        // resident rounds | one block per tile), then tiles of 256 and 512 cells at two cells
+        // per thread (one block per tile)
         double best_x = 1e300;
-        int grid_x = 0, tile_x = 256;
+        int grid_x = 0, tile_x = 256, cpt_x = 1;
         const int run0 = b->tuning.y_run_rows;
+        b->tuning.x_cells_per_thread = 1;
         for (int tile : kBatchTileWidths) {
             b->tuning.tile_x = tile;
             build_items(b, run0);
@@ -641,7 +652,16 @@ ARMON_EXPORT int armon_batch_autotune(ArmonBatch* b, ArmonBatchTuning* out)
                 if (ms < best_x) { best_x = ms; grid_x = g; tile_x = tile; }
             }
         }
+        b->tuning.x_cells_per_thread = 2;
+        b->tuning.grid_x = 0;
+        for (int tile : {256, 512}) {
+            b->tuning.tile_x = tile;
+            build_items(b, run0);
+            const double ms = time_sweep(b, ARMON_AXIS_X, reps);
+            if (ms < best_x) { best_x = ms; grid_x = 0; tile_x = tile; cpt_x = 2; }
+        }
         b->tuning.tile_x = tile_x;
+        b->tuning.x_cells_per_thread = cpt_x;
         b->tuning.grid_x = grid_x;
 
         // Y: run length x (one resident round of blocks | one block per 4 runs)

@@ -272,7 +272,8 @@ ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
         const T a_urho = dX * rho * u - (f_urho_ip - f_urho_i);
         const T a_vrho = dX * rho * v - (f_vrho_ip - f_vrho_i);
         const T a_Erho = dX * rho * E - (f_Erho_ip - f_Erho_i);
-        if (recip_div_ok(a_rho) & recip_div_ok(a_urho) & recip_div_ok(a_vrho) & recip_div_ok(a_Erho)) {
+        if ((int) recip_div_ok(a_rho) & (int) recip_div_ok(a_urho) & (int) recip_div_ok(a_vrho) &
+            (int) recip_div_ok(a_Erho)) {
             n_rho = div_by_recip(a_rho, dx, inv_dx);
             n_urho = div_by_recip(a_urho, dx, inv_dx);
             n_vrho = div_by_recip(a_vrho, dx, inv_dx);

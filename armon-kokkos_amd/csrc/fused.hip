@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_eos_real(const T* rho, const T* u, cons
 
 template void launch_sweep<double>(int, const SweepVariant&, const SweepArgs<double>&, hipStream_t);
 template void launch_batch_sweep<double>(int, const SweepVariant&, const BatchArgs<double>&, const void*, int, int, int,
-                                       hipStream_t);
+                                       int, hipStream_t);
 template int batch_resident_blocks<double>(int, const SweepVariant&, int);
 
 int x_tile_outputs(const SweepVariant& v, int tile) { return tile - 2 * read_extent(v.flux, v.proj); }

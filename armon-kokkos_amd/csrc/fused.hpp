@@ -91,11 +91,12 @@ struct BatchArgs {
 };
 
 // One launch sweeps every domain of the batch along `axis`: `grid` persistent blocks walk
-// the work items (BatchItemX for X, BatchItemY for Y); X tiles are `tile` threads wide (128,
-// 256 or 512, the width the X items were cut for).
+// the work items (BatchItemX for X, BatchItemY for Y); X tiles are `tile` cells wide (128,
+// 256 or 512, the width the X items were cut for), `cpt` cells per thread (1, or 2 for tiles of
+// 256 or 512 cells, one block per tile whatever `grid`).
 template <class T>
 void launch_batch_sweep(int axis, const SweepVariant& v, const BatchArgs<T>& b, const void* items, int nitems,
-                        int grid, int tile, hipStream_t s);
+                        int grid, int tile, int cpt, hipStream_t s);
 
 // Blocks of the batched sweep kernel of this variant (X tiles of `tile` threads) that fit on the
 // device at once.

@@ -11,7 +11,7 @@ namespace armon {
 
 template void launch_sweep<float>(int, const SweepVariant&, const SweepArgs<float>&, hipStream_t);
 template void launch_batch_sweep<float>(int, const SweepVariant&, const BatchArgs<float>&, const void*, int, int, int,
-                                       hipStream_t);
+                                      int, hipStream_t);
 template int batch_resident_blocks<float>(int, const SweepVariant&, int);
 
 }  // namespace armon

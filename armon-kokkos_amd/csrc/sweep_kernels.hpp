@@ -8,8 +8,8 @@
 //
 //  * X sweep (k_sweep_x): one 256-thread block per tile of a stream of padded rows. Each
 //    thread owns one cell of the tile (outputs plus an R-cell halo on each side, R = ghost
-//    layers read); the stage results are exchanged through LDS between barriers. Loads and
-//    stores are contiguous.
+//    layers read; the batched sweeps can also give each thread 2 cells); the stage results are
+//    exchanged through LDS between barriers. Loads and stores are contiguous.
 //  * Y sweep (k_sweep_y): one thread per column, marching down a run of rows. Every
 //    stage's stencil neighbours along y live in the thread's own registers (a sliding
 //    window), so no LDS and no barriers; consecutive lanes touch consecutive columns.
@@ -238,8 +238,9 @@ struct SmemX {
     __device__ T (&f())[4][TILE] { return x; }
 };
 
-// One X tile of the domain described by `a`, with the cycle's dt. Called by all threads of a
-// TILE-thread block.
+// One X tile of the domain described by `a`, with the cycle's dt. Called by all TILE / CPT
+// threads of a block; each thread computes CPT cells of the tile, at slots t, t + TILE / CPT, ...
+// (consecutive lanes on consecutive slots for every load and store).
 //
 // The rows of the domain are laid end to end as a stream of slots, each row taking
 // x_slots = nx + 2R slots (its R ghost cells on each side, then the next row). A tile covers
@@ -247,112 +248,164 @@ struct SmemX {
 // slots, and the slots [R, TILE - R) of a tile that hold real cells are its outputs. A cell's
 // result depends only on the cells within R of it in its own row, so a tile may span the end
 // of one row and the start of the next; no thread is spent on a tile's share of a row end.
-template <class T, int FLUX, int PROJ, int EOS, int DXM, int TILE = kTile>
+template <class T, int FLUX, int PROJ, int EOS, int DXM, int TILE = kTile, int CPT = 1>
 __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0, int k0, SmemX<T, TILE>& sm)
 {
     constexpr int GAD = FLUX > 0 ? 1 : 0;
     constexpr int E2 = PROJ == 2 ? 1 : 0;
     constexpr int R = 2 + GAD + E2;
     constexpr int W = TILE - 2 * R;
+    constexpr int NT = TILE / CPT;
+    static_assert(TILE % CPT == 0, "a tile is a whole number of cells per thread");
     const T dx = (T) a.dx;
 
-    const int t = threadIdx.x;
-    // row and cell of this thread's slot: n = k0 + t < x_slots + TILE. Rows of at least TILE
-    // slots: n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and
-    // a single correction makes it exact.
-    const int n = k0 + t;
-    int dj, r;
-    if (a.x_slots >= TILE) {
-        dj = n >= a.x_slots ? 1 : 0;
-        r = n - (dj ? a.x_slots : 0);
-    } else {
-        dj = (int) ((float) n * a.x_inv_slots);
-        r = n - dj * a.x_slots;
-        if (r < 0) { dj--; r += a.x_slots; }
-        else if (r >= a.x_slots) { dj++; r -= a.x_slots; }
+    int slot[CPT], jc[CPT], kc[CPT];
+    bool in_range[CPT];
+    Cell<T> cell[CPT];
+    // Stage 0: load (or synthesise) the cells, EOS.
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int t = threadIdx.x + c * NT;
+        slot[c] = t;
+        // row and cell of the slot: n = k0 + t < x_slots + TILE. Rows of at least TILE slots:
+        // n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and a
+        // single correction makes it exact.
+        const int n = k0 + t;
+        int dj, r;
+        if (a.x_slots >= TILE) {
+            dj = n >= a.x_slots ? 1 : 0;
+            r = n - (dj ? a.x_slots : 0);
+        } else {
+            dj = (int) ((float) n * a.x_inv_slots);
+            r = n - dj * a.x_slots;
+            if (r < 0) { dj--; r += a.x_slots; }
+            else if (r >= a.x_slots) { dj++; r -= a.x_slots; }
+        }
+        jc[c] = j0 + dj;
+        kc[c] = r - R;                    // -R <= k < nx + R
+        in_range[c] = jc[c] < a.ny;       // slots past the last row
+        cell[c] = in_range[c] ? load_cell<T, 0, EOS>(a, jc[c], kc[c], a.nx) : benign_cell<T>();
+        sm.rho[t] = cell[c].rho;
+        sm.un[t] = cell[c].un;
+        sm.p()[t] = cell[c].p;
+        sm.c()[t] = cell[c].c;
     }
-    const int j = j0 + dj;
-    const int k = r - R;                  // -R <= k < nx + R
-    const bool in_range = j < a.ny;       // slots past the last row
-    const int tm = t > 0 ? t - 1 : 0;
-    const int tp = t < TILE - 1 ? t + 1 : TILE - 1;
-
-    // Stage 0: load (or synthesise) the cell, EOS.
-    const Cell<T> cell = in_range ? load_cell<T, 0, EOS>(a, j, k, a.nx) : benign_cell<T>();
-    sm.rho[t] = cell.rho;
-    sm.un[t] = cell.un;
-    sm.p()[t] = cell.p;
-    sm.c()[t] = cell.c;
     __syncthreads();
 
     // Stage 1-2: face flux at face k (between cells k-1 and k).
-    T gu, gp, us, ps;
-    godunov<T>(cell.rho, sm.rho[tm], cell.c, sm.c()[tm], cell.un, sm.un[tm], cell.p, sm.p()[tm], gu, gp);
-    if constexpr (GAD) {
-        sm.gu()[t] = gu;
-        sm.gp()[t] = gp;
-        __syncthreads();
-        Cell<T> m;
-        m.rho = sm.rho[tm]; m.un = sm.un[tm]; m.p = sm.p()[tm]; m.c = sm.c()[tm];
-        face_flux_gad<T, FLUX - 1>(dt, dx, m, cell, sm.gu()[tm], sm.gp()[tm], gu, gp, sm.gu()[tp], sm.gp()[tp], us, ps);
-    } else {
-        us = gu;
-        ps = gp;
+    T us[CPT], ps[CPT];
+    {
+        T gu[CPT], gp[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int t = slot[c], tm = t > 0 ? t - 1 : 0;
+            godunov<T>(cell[c].rho, sm.rho[tm], cell[c].c, sm.c()[tm], cell[c].un, sm.un[tm], cell[c].p, sm.p()[tm],
+                       gu[c], gp[c]);
+        }
+        if constexpr (GAD) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                sm.gu()[slot[c]] = gu[c];
+                sm.gp()[slot[c]] = gp[c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int t = slot[c], tm = t > 0 ? t - 1 : 0, tp = t < TILE - 1 ? t + 1 : TILE - 1;
+                Cell<T> m;
+                m.rho = sm.rho[tm]; m.un = sm.un[tm]; m.p = sm.p()[tm]; m.c = sm.c()[tm];
+                face_flux_gad<T, FLUX - 1>(dt, dx, m, cell[c], sm.gu()[tm], sm.gp()[tm], gu[c], gp[c], sm.gu()[tp],
+                                           sm.gp()[tp], us[c], ps[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                us[c] = gu[c];
+                ps[c] = gp[c];
+            }
+        }
     }
-    sm.us[t] = us;
-    sm.ps[t] = ps;
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        sm.us[slot[c]] = us[c];
+        sm.ps[slot[c]] = ps[c];
+    }
     __syncthreads();
 
     // Stage 3: Lagrangian update of cell k from faces k and k+1.
-    T rho = cell.rho, un = cell.un, ut = cell.ut, E = cell.E;
-    const T us_p = sm.us[tp];
-    cell_update<T>(dx, dt, us, us_p, ps, sm.ps[tp], rho, un, E);
-    sm.rho[t] = rho;
-    sm.un[t] = un;
-    sm.ut[t] = ut;
-    sm.E[t] = E;
+    T rho[CPT], un[CPT], ut[CPT], E[CPT], us_p[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int t = slot[c], tp = t < TILE - 1 ? t + 1 : TILE - 1;
+        rho[c] = cell[c].rho; un[c] = cell[c].un; ut[c] = cell[c].ut; E[c] = cell[c].E;
+        us_p[c] = sm.us[tp];
+        cell_update<T>(dx, dt, us[c], us_p[c], ps[c], sm.ps[tp], rho[c], un[c], E[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int t = slot[c];
+        sm.rho[t] = rho[c];
+        sm.un[t] = un[c];
+        sm.ut[t] = ut[c];
+        sm.E[t] = E[c];
+    }
     __syncthreads();
 
     // Stage 4: advection flux at face k.
-    T f0, f1, f2, f3;
-    const T disp = dt * us;
-    if constexpr (E2) {
-        T Dx;
-        int td;
-        if (disp > 0) {
-            Dx = -(dx - dt * sm.us[tm]);
-            td = tm;
+    T f[CPT][4];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int t = slot[c], tm = t > 0 ? t - 1 : 0, tp = t < TILE - 1 ? t + 1 : TILE - 1;
+        const T disp = dt * us[c];
+        if constexpr (E2) {
+            T Dx;
+            int td;
+            if (disp > 0) {
+                Dx = -(dx - dt * sm.us[tm]);
+                td = tm;
+            } else {
+                Dx = dx + dt * sm.us[tp];
+                td = t;
+            }
+            const int tdm = td > 0 ? td - 1 : 0;
+            const int tdp = td < TILE - 1 ? td + 1 : TILE - 1;
+            const int tdpp = td < TILE - 2 ? td + 2 : TILE - 1;
+            advection_flux_2<T>(dx, dt, disp, Dx, sm.us[tdm], sm.us[td], sm.us[tdp], sm.us[tdpp], sm.rho[tdm],
+                                sm.rho[td], sm.rho[tdp], sm.un[tdm], sm.un[td], sm.un[tdp], sm.ut[tdm], sm.ut[td],
+                                sm.ut[tdp], sm.E[tdm], sm.E[td], sm.E[tdp], f[c][0], f[c][1], f[c][2], f[c][3]);
         } else {
-            Dx = dx + dt * sm.us[tp];
-            td = t;
+            const int td = disp > 0 ? tm : t;
+            advection_flux_1<T>(disp, sm.rho[td], sm.un[td], sm.ut[td], sm.E[td], f[c][0], f[c][1], f[c][2], f[c][3]);
         }
-        const int tdm = td > 0 ? td - 1 : 0;
-        const int tdp = td < TILE - 1 ? td + 1 : TILE - 1;
-        const int tdpp = td < TILE - 2 ? td + 2 : TILE - 1;
-        advection_flux_2<T>(dx, dt, disp, Dx, sm.us[tdm], sm.us[td], sm.us[tdp], sm.us[tdpp], sm.rho[tdm], sm.rho[td],
-                            sm.rho[tdp], sm.un[tdm], sm.un[td], sm.un[tdp], sm.ut[tdm], sm.ut[td], sm.ut[tdp], sm.E[tdm],
-                            sm.E[td], sm.E[tdp], f0, f1, f2, f3);
-    } else {
-        const int td = disp > 0 ? tm : t;
-        advection_flux_1<T>(disp, sm.rho[td], sm.un[td], sm.ut[td], sm.E[td], f0, f1, f2, f3);
     }
-    sm.f()[0][t] = f0;
-    sm.f()[1][t] = f1;
-    sm.f()[2][t] = f2;
-    sm.f()[3][t] = f3;
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) sm.f()[q][slot[c]] = f[c][q];
+    }
     __syncthreads();
 
     // Stage 5: projection of cell k, output.
     double local = DBL_MAX;
-    if (t >= R && t < R + W && in_range && k >= 0 && k < a.nx) {
-        projection<DXM, T>(dx, (T) a.inv_dx, dt, us, us_p, f0, sm.f()[0][tp], f1, sm.f()[1][tp], f2, sm.f()[2][tp], f3,
-                           sm.f()[3][tp], rho, un, ut, E);
-        const int idx = a.index_start + j * a.row_length + k;
-        a.out_rho[idx] = rho;
-        a.out_u[idx] = un;
-        a.out_v[idx] = ut;
-        a.out_E[idx] = E;
-        if (a.reduce) local = cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) un, (double) ut, (double) cell.c, 1.0);
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int t = slot[c], tp = t < TILE - 1 ? t + 1 : TILE - 1;
+        if (t >= R && t < R + W && in_range[c] && kc[c] >= 0 && kc[c] < a.nx) {
+            projection<DXM, T>(dx, (T) a.inv_dx, dt, us[c], us_p[c], f[c][0], sm.f()[0][tp], f[c][1], sm.f()[1][tp],
+                               f[c][2], sm.f()[2][tp], f[c][3], sm.f()[3][tp], rho[c], un[c], ut[c], E[c]);
+            const int idx = a.index_start + jc[c] * a.row_length + kc[c];
+            a.out_rho[idx] = rho[c];
+            a.out_u[idx] = un[c];
+            a.out_v[idx] = ut[c];
+            a.out_E[idx] = E[c];
+            if (a.reduce) {
+                const double v = cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) un[c], (double) ut[c],
+                                                  (double) cell[c].c, 1.0);
+                local = any ? kmin(local, v) : v;
+                any = true;
+            }
+        }
     }
     if (a.reduce) reduce_cfl(a.st, local);
 }
@@ -605,17 +658,18 @@ __global__ void __launch_bounds__(TILE) k_batch_x(BatchArgs<T> b, const BatchIte
 }
 
 // One block per X item (the launch configuration the autotuner usually picks): no item loop,
-// so the tile body keeps the single-domain kernel's register budget.
-template <class T, int FLUX, int PROJ, int EOS, int TILE>
-__global__ void __launch_bounds__(TILE) k_batch_x1(BatchArgs<T> b, const BatchItemX* __restrict__ items)
+// so the tile body keeps the single-domain kernel's register budget. TILE cells per tile,
+// CPT cells per thread.
+template <class T, int FLUX, int PROJ, int EOS, int TILE, int CPT = 1>
+__global__ void __launch_bounds__(TILE / CPT) k_batch_x1(BatchArgs<T> b, const BatchItemX* __restrict__ items)
 {
     __shared__ SmemX<T, TILE> sm;
     const BatchItemX item = items[xcd_block(blockIdx.x, gridDim.x)];
     const DeviceState* st = b.states + item.domain;
     if (!st->active) return;
     const SweepArgs<T> a = batch_domain_args(b, item.domain, 0);
-    sweep_x_tile<T, FLUX, PROJ, EOS, kBatchDxMode<T>, TILE>(a, (T) (st->current_dt * b.dt_factor), item.row,
-                                                           item.slot, sm);
+    sweep_x_tile<T, FLUX, PROJ, EOS, kBatchDxMode<T>, TILE, CPT>(a, (T) (st->current_dt * b.dt_factor), item.row,
+                                                                item.slot, sm);
 }
 
 // The row-ahead loads of the FP64 march need 179 VGPRs here (the per-item domain arguments
@@ -795,17 +849,32 @@ void launch_batch_x(const BatchArgs<T>& b, const BatchItemX* items, int nitems, 
         k_batch_x<T, F, P, E, TILE><<<dim3((unsigned) grid), dim3(TILE), 0, s>>>(b, items, nitems);
 }
 
+// Two cells per thread: one block per tile of TILE cells (TILE / 2 threads).
+template <class T, int F, int P, int E, int TILE>
+void launch_batch_x2(const BatchArgs<T>& b, const BatchItemX* items, int nitems, hipStream_t s)
+{
+    k_batch_x1<T, F, P, E, TILE, 2><<<dim3((unsigned) nitems), dim3(TILE / 2), 0, s>>>(b, items);
+}
+
+// X: `tile` cells per tile (128, 256 or 512), `cpt` cells per thread (1; or 2 for tiles of 256
+// or 512 cells, one block per tile); `grid` blocks, or one per tile when grid >= nitems.
 template <class T, int F, int P, int E>
-void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, int nitems, int grid, int tile,
+void launch_batch_variant(int axis, const BatchArgs<T>& b, const void* items, int nitems, int grid, int tile, int cpt,
                           hipStream_t s)
 {
     if (nitems <= 0) return;
     if (axis == 0) {
         const auto* x_items = static_cast<const BatchItemX*>(items);
-        switch (tile) {
-        case 128: launch_batch_x<T, F, P, E, 128>(b, x_items, nitems, grid, s); break;
-        case 512: launch_batch_x<T, F, P, E, 512>(b, x_items, nitems, grid, s); break;
-        default: launch_batch_x<T, F, P, E, 256>(b, x_items, nitems, grid, s); break;
+        if (cpt == 2) {
+            if (tile == 512) launch_batch_x2<T, F, P, E, 512>(b, x_items, nitems, s);
+            else if (tile == 256) launch_batch_x2<T, F, P, E, 256>(b, x_items, nitems, s);
+            else throw std::invalid_argument("2 cells per thread needs X tiles of 256 or 512 cells");
+        } else {
+            switch (tile) {
+            case 128: launch_batch_x<T, F, P, E, 128>(b, x_items, nitems, grid, s); break;
+            case 512: launch_batch_x<T, F, P, E, 512>(b, x_items, nitems, grid, s); break;
+            default: launch_batch_x<T, F, P, E, 256>(b, x_items, nitems, grid, s); break;
+            }
         }
     } else {
         constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
@@ -871,10 +940,10 @@ void launch_sweep(int axis, const SweepVariant& v, const SweepArgs<T>& a, hipStr
 
 template <class T>
 void launch_batch_sweep(int axis, const SweepVariant& v, const BatchArgs<T>& b, const void* items, int nitems, int grid,
-                        int tile, hipStream_t s)
+                        int tile, int cpt, hipStream_t s)
 {
     sweep::with_variant(v, [&]<int F, int P, int E>() {
-        sweep::launch_batch_variant<T, F, P, E>(axis, b, items, nitems, grid, tile, s);
+        sweep::launch_batch_variant<T, F, P, E>(axis, b, items, nitems, grid, tile, cpt, s);
         return 0;
     });
 }

@@ -1,6 +1,6 @@
 // Stage tracing of the host driver.
 //
-//  - roctx ranges named as the reference's NVTX ranges (armon_2D.cpp:18-54, 306-322, 339, 394-404):
+//  - roctx ranges named as the reference's NVTX ranges (armon_2D.cpp:19-54, 306-322, 339, 394-404):
 //    cycle, time_step, axis, EOS, BC, fluxes, update, remap, EOS_init, init_test. They cost
 //    nothing unless a tool listens (rocprofv3 --marker-trace).
 //  - a per-stage time table, the counterpart of the reference's TIC/TAC table

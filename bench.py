@@ -11,11 +11,12 @@ roofline of the dominant kernel (timed with HIP events on the solver stream), an
 baseline (the oracle's fast-math build on this host's cores, on a bounded sample).
 
 Beside the headline:
-- `developed_flow`: the circular Sod of radius 0.2 at 8192^2 run 200 cycles in FP64 and in
-  FP32-mixed from the same initial state, the error of every FP32-mixed field against the FP64
-  state after those cycles (the FP64 state is the oracle's, bit for bit, by the golden digests
-  of the test suite), then the rate of both forms timed from that developed state, where most
-  cells are in motion (the headline Sedov is near quiescent over its first cycles);
+- `developed_flow`: the circular Sod of radius 0.2 at 8192^2, set in a vortex of speed 0.5 so
+  that every cell moves, in every direction (the headline Sedov moves only the cells its blast
+  has reached), run 200 cycles in FP64 and in FP32-mixed from the same initial state; the error
+  of every FP32-mixed field against the FP64 state after those cycles (the FP64 state is the
+  oracle's, bit for bit, by the golden digests of the test suite), then the rate of both forms
+  timed from that developed state;
 - `batched`: the 256-domain ensemble, FP64 and FP32-mixed.
 """
 from __future__ import annotations
@@ -78,6 +79,24 @@ def sod_circ_fields(p, radius):
     n = p.nb_cells
     return {"rho": np.where(inside, 1.0, 0.125), "umat": np.zeros(n), "vmat": np.zeros(n),
             "Emat": np.where(inside, 2.5, 2.0)}
+
+
+def swirl_fields(p, radius, speed):
+    """The circular Sod state of sod_circ_fields set in a vortex of speed `speed`:
+    u = speed sin(pi x) cos(pi y), v = -speed cos(pi x) sin(pi y) over the unit square (zero
+    normal to its reflective walls), E raised by the kinetic energy so that the pressure is the
+    test's. Every cell moves from the first cycle, in every direction."""
+    f = sod_circ_fields(p, radius)
+    dx, dy = p.domain_size[0] / p.nx, p.domain_size[1] / p.ny
+    g = p.nb_ghosts
+    x = p.domain_origin[0] + (np.arange(p.row_length) - g + 0.5) * dx
+    y = p.domain_origin[1] + (np.arange(p.col_length) - g + 0.5) * dy
+    n = p.nb_cells
+    u = (speed * np.sin(np.pi * x)[None, :] * np.cos(np.pi * y)[:, None]).reshape(-1)[:n]
+    v = (-speed * np.cos(np.pi * x)[None, :] * np.sin(np.pi * y)[:, None]).reshape(-1)[:n]
+    f["umat"], f["vmat"] = u, v
+    f["Emat"] = f["Emat"] + 0.5 * (u * u + v * v)
+    return f
 
 
 def field_errors(a, b):
@@ -267,15 +286,15 @@ def main():
 
     developed = None
     if not args.no_developed:
-        # circular Sod of radius 0.2: 200 cycles in both forms from the same state, the FP32-mixed
-        # error against the FP64 state, then both rates timed from the developed state
+        # circular Sod of radius 0.2 in a vortex: 200 cycles in both forms from the same state, the
+        # FP32-mixed error against the FP64 state, then both rates timed from the developed state
         dev = {}
         states = {}
         for prec, key in ((0, "fp64"), (1, "fp32_mixed")):
-            log(f"[bench] developed flow {key}: circular Sod r=0.2 {n}x{n}, {args.develop_cycles} cycles, then "
-                f"{args.steps} timed")
+            log(f"[bench] developed flow {key}: circular Sod r=0.2 in a vortex {n}x{n}, {args.develop_cycles} "
+                f"cycles, then {args.steps} timed")
             el, kt, st = measure_gpu(armon, n, n, args.steps, args.warmup, prec, 10, chunk, test="Sod_circ",
-                                     init_fields=lambda p: sod_circ_fields(p, 0.2), develop=args.develop_cycles)
+                                     init_fields=lambda p: swirl_fields(p, 0.2, 0.5), develop=args.develop_cycles)
             states[key] = st
             dev[key] = {"value": cells * args.steps / el, "ms_per_step": el / args.steps * 1e3,
                         "sweep_ms": {names[k]: kt.total_ms[k] / max(kt.launches[k], 1) for k in range(3)}}
@@ -287,7 +306,8 @@ def main():
         dev["fp32_mixed"]["errors_vs_fp64"] = errors
         dev["fp64_vs_headline"] = dev["fp64"]["value"] / rate64
         dev["moving_cell_fraction"] = moving
-        dev["workload"] = (f"circular Sod, radius 0.2 about (0.5, 0.5), {n}x{n}, same scheme and CFL as the "
+        dev["workload"] = (f"circular Sod, radius 0.2 about (0.5, 0.5), in the vortex u = 0.5 sin(pi x) cos(pi y), "
+                           f"v = -0.5 cos(pi x) sin(pi y), {n}x{n}, same scheme and CFL as the "
                            f"headline; FP64 and FP32-mixed each run {args.develop_cycles} cycles from the same "
                            f"initial state, errors taken there; then {args.warmup} warm-up and {args.steps} timed "
                            f"cycles from that state")
@@ -296,7 +316,7 @@ def main():
             fp32["rho_rel_L1_vs_fp64"] = errors["rho"]["rel_L1"]
             fp32["rho_Linf_vs_fp64"] = errors["rho"]["Linf"]
             fp32["errors_vs_fp64"] = errors
-            fp32["errors_from"] = f"developed_flow ({args.develop_cycles} cycles of circular Sod r=0.2)"
+            fp32["errors_from"] = f"developed_flow ({args.develop_cycles} cycles of circular Sod r=0.2 in a vortex)"
         log("[bench] FP32-mixed vs FP64 after the developed run: " +
             ", ".join(f"{f} " + " / ".join(f"{k} {v:.2e}" for k, v in e.items()) for f, e in errors.items()))
         log(f"[bench] developed flow: {moving:.0%} of cells moving; FP64 {dev['fp64_vs_headline']:.3f}x the "
@@ -316,11 +336,13 @@ def main():
                 "value": rate, "unit": "cell-updates/s", "ms_per_step": el / args.steps * 1e3,
                 "sweep_ms": avg,
                 "sweep_GBps": {k: 8 * word * bcells / (avg[k] * 1e-3) / 1e9 for k in ("sweep_x", "sweep_y")},
-                "tuning": {"tile_x": tun.tile_x, "y_run_rows": tun.y_run_rows, "grid_x": tun.grid_x,
-                           "grid_y": tun.grid_y, "items_x": tun.items_x, "items_y": tun.items_y},
+                "tuning": {"tile_x": tun.tile_x, "x_cells_per_thread": tun.x_cells_per_thread,
+                           "y_run_rows": tun.y_run_rows, "grid_x": tun.grid_x, "grid_y": tun.grid_y,
+                           "items_x": tun.items_x, "items_y": tun.items_y},
             }
             log(f"[bench] batched {key}: {rate / 1e9:.3f} G cell-updates/s over {bcells} cells, "
-                f"{el / args.steps * 1e3:.3f} ms/cycle, tuning tile {tun.tile_x} L={tun.y_run_rows} gx={tun.grid_x} "
+                f"{el / args.steps * 1e3:.3f} ms/cycle, tuning tile {tun.tile_x}x{tun.x_cells_per_thread} "
+                f"L={tun.y_run_rows} gx={tun.grid_x} "
                 f"gy={tun.grid_y}")
         batched["domains"] = args.batch_domains
         batched["cells"] = bcells

@@ -276,15 +276,17 @@ ARMON_EXPORT int armon_batch_random_specs(uint64_t seed, int n, int64_t min_area
                                           double max_aspect, ArmonDomainSpec* out);
 
 /* Launch configuration of the batched sweeps, chosen by armon_batch_autotune: the tile shape
- * and the cells each thread computes (an X tile is tile_x threads of one cell each along a row;
- * a Y item is a wave of 64 columns, each thread marching y_run_rows cells down its column) and
- * the launch grids. */
+ * and the cells each thread computes (an X tile is tile_x cells along a row, computed by
+ * tile_x / x_cells_per_thread threads; a Y item is a wave of 64 columns, each thread marching
+ * y_run_rows cells down its column) and the launch grids. */
 typedef struct ArmonBatchTuning {
     int y_run_rows;           /* rows a wave marches per Y work item (64 columns wide) */
     int grid_x, grid_y;       /* blocks of the X and Y launches (persistent when below the item count) */
     int items_x, items_y;     /* work items per X / Y sweep (outputs) */
     double ms_x, ms_y;        /* measured time per X / Y sweep of this configuration (autotune output) */
-    int tile_x;               /* threads (cells) per X tile: 128, 256 or 512 (0 in set_tuning: 256) */
+    int tile_x;               /* cells per X tile: 128, 256 or 512 (0 in set_tuning: 256) */
+    int x_cells_per_thread;   /* cells per thread of the X tile: 1, or 2 for tiles of 256 or 512 cells
+                                 with one block per tile (grid_x <= 0); 0 in set_tuning: 1 */
 } ArmonBatchTuning;
 
 typedef struct ArmonBatch ArmonBatch;

@@ -95,6 +95,26 @@ def sod_circ_radius_fields(p, radius):
     return {"rho": rho, "umat": np.zeros(n), "vmat": np.zeros(n), "Emat": E}
 
 
+def sod_circ_swirl_fields(p, radius, speed):
+    """The circular Sod state of sod_circ_radius_fields set in a vortex: u = speed sin(pi x)
+    cos(pi y), v = -speed cos(pi x) sin(pi y) over the unit square, which vanishes normal to its
+    reflective walls; E adds the kinetic energy, so the pressure stays the test's. Every cell
+    moves from the first cycle, in every direction."""
+    import numpy as np
+
+    f = sod_circ_radius_fields(p, radius)
+    dx, dy = p.domain_size[0] / p.nx, p.domain_size[1] / p.ny
+    g = p.nb_ghosts
+    x = p.domain_origin[0] + (np.arange(p.row_length) - g + 0.5) * dx
+    y = p.domain_origin[1] + (np.arange(p.col_length) - g + 0.5) * dy
+    n = p.nb_cells
+    u = (speed * np.sin(np.pi * x)[None, :] * np.cos(np.pi * y)[:, None]).reshape(-1)[:n]
+    v = (-speed * np.cos(np.pi * x)[None, :] * np.sin(np.pi * y)[:, None]).reshape(-1)[:n]
+    f["umat"], f["vmat"] = u, v
+    f["Emat"] = f["Emat"] + 0.5 * (u * u + v * v)
+    return f
+
+
 def bizarrium_fields(p, left_moving):
     """Initial rho, umat, vmat, Emat of the Bizarrium tube over all nb_cells of params `p`: the
     reference's (init_test.cpp:25-35: x <= 0.5 at rho 1.42857142857e4 and internal energy

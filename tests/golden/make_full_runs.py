@@ -11,6 +11,9 @@ Cases (GAD, Euler 2nd-order remap, Sequential splitting, 4 ghost layers):
   t = 1;
 - circular Sod of radius 0.2 (the benchmark list's; the reference's is 0.3) at 8192x8192,
   minmod, CFL 0.7, 200 steps, through the initial fields of tests/conftest.py;
+- the same circular Sod set in a vortex of speed 0.5 (tests/conftest.py sod_circ_swirl_fields:
+  every cell moving from the first cycle; the benchmark's developed flow) at 8192x8192, minmod,
+  CFL 0.7, 200 steps;
 - the Bizarrium tube with its left state moving at u = 250 (the benchmark list's; the reference
   moves the right one) at 4096x4096, superbee, CFL 0.6, 500 steps;
 - the batched ensemble of the benchmark: 256 domains drawn from seed 2024 by the library's
@@ -51,6 +54,9 @@ CASES = {
     "sod_circ_r02_8192_200_steps": dict(test="Sod_circ", init="sod_circ_r02",
                                         params=dict(_COMMON, limiter=1, cfl=0.7, max_time=1e9, nx=8192, ny=8192,
                                                     max_cycles=200)),
+    "sod_circ_r02_swirl_8192_200_steps": dict(test="Sod_circ", init="sod_circ_r02_swirl",
+                                              params=dict(_COMMON, limiter=1, cfl=0.7, max_time=1e9, nx=8192,
+                                                          ny=8192, max_cycles=200)),
     "bizarrium_left_moving_4096_superbee_500_steps": dict(
         test="Bizarrium", init="bizarrium_left_moving",
         params=dict(_COMMON, limiter=2, cfl=0.6, max_time=1.0, nx=4096, ny=4096, max_cycles=500)),
@@ -70,12 +76,14 @@ def initial_fields(case: dict, p):
     if init is None:
         return None
     sys.path.insert(0, os.path.join(HERE, ".."))
-    from conftest import bizarrium_fields, sedov_corner_fields, sod_circ_radius_fields
+    from conftest import bizarrium_fields, sedov_corner_fields, sod_circ_radius_fields, sod_circ_swirl_fields
 
     if init == "sedov_corner":
         return sedov_corner_fields(p)
     if init == "sod_circ_r02":
         return sod_circ_radius_fields(p, 0.2)
+    if init == "sod_circ_r02_swirl":
+        return sod_circ_swirl_fields(p, 0.2, 0.5)
     if init == "bizarrium_left_moving":
         return bizarrium_fields(p, left_moving=True)
     raise ValueError(init)

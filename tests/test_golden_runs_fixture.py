@@ -65,12 +65,14 @@ def test_sedov_mass_leaves_only_through_the_boundary(oracle, po):
 
 
 def test_sod_circ_conserves_mass():
-    """After 200 steps at 8192^2 the circular Sod waves are far from the boundary, so the mass
-    is the initial one to rounding: cells whose centre lies within the radius (0.3, the
-    reference's; 0.2 for the benchmark list's variant) of (0.5, 0.5) hold rho = 1, the others
+    """After 200 steps at 8192^2 the circular Sod waves are far from the boundary (and the
+    vortex of the swirling variant is tangent to the reflective walls), so the mass is the
+    initial one to rounding: cells whose centre lies within the radius (0.3, the reference's;
+    0.2 for the benchmark list's variants) of (0.5, 0.5) hold rho = 1, the others
     0.125 (test_cases.h:73-76, init_test.cpp:12-22), counted here from the cell centres without
     the oracle."""
-    for name, radius in (("sod_circ_8192_200_steps", 0.3), ("sod_circ_r02_8192_200_steps", 0.2)):
+    for name, radius in (("sod_circ_8192_200_steps", 0.3), ("sod_circ_r02_8192_200_steps", 0.2),
+                         ("sod_circ_r02_swirl_8192_200_steps", 0.2)):
         c = load_cases()[name]
         n = c["params"]["nx"]
         centre = (np.arange(n) + 0.5) / n
@@ -90,11 +92,13 @@ def test_positive_fields():
 
 
 def test_symmetric_blasts_have_no_net_velocity():
-    """The Sedov and circular Sod states are mirror-symmetric in x and y, so each velocity
+    """The Sedov and circular Sod states are mirror-symmetric in x and y, and the swirling one
+    is symmetric under the half turn about (0.5, 0.5) (which negates u and v), so each velocity
     component sums to zero up to rounding (not exactly: the splitting evaluates the two sides
     of a mirror in different orders)."""
     cases = load_cases()
-    for name in ("sedov_1024_to_end_time", "sod_circ_8192_200_steps", "sod_circ_r02_8192_200_steps"):
+    for name in ("sedov_1024_to_end_time", "sod_circ_8192_200_steps", "sod_circ_r02_8192_200_steps",
+                 "sod_circ_r02_swirl_8192_200_steps"):
         for f in ("umat", "vmat"):
             d = cases[name]["fields"][f]
             assert d["abs_sum"] > 0 and abs(d["sum"]) <= 1e-4 * d["abs_sum"], (name, f, d)
@@ -134,3 +138,30 @@ def test_corner_sedov_keeps_its_mass():
     c = load_cases()["sedov_corner_1024_to_end_time"]
     m0 = float(c["params"]["nx"] * c["params"]["ny"])
     assert abs(c["fields"]["rho"]["sum"] - m0) <= 1e-12 * m0
+
+
+def test_benchmark_vortex_state_is_the_fixture_state():
+    """bench.py's developed flow starts from the initial state of the swirling circular Sod
+    case (the same arrays, bit for bit), so the fixture's digests pin its FP64 side; the
+    vortex is tangent to the walls and the pressure is the test's."""
+    import types
+
+    sys.path.insert(0, os.path.dirname(HERE))
+    import bench
+    from conftest import sod_circ_swirl_fields
+
+    n, g = 96, 4
+    p = types.SimpleNamespace(nx=n, ny=n, nb_ghosts=g, row_length=n + 2 * g, col_length=n + 2 * g,
+                              nb_cells=(n + 2 * g) ** 2, domain_origin=(0.0, 0.0), domain_size=(1.0, 1.0))
+    a, b = bench.swirl_fields(p, 0.2, 0.5), sod_circ_swirl_fields(p, 0.2, 0.5)
+    for f in ("rho", "umat", "vmat", "Emat"):
+        assert np.array_equal(a[f], b[f]), f
+    rl = p.row_length
+    u = a["umat"].reshape(-1, rl)[g:g + n, g:g + n]
+    v = a["vmat"].reshape(-1, rl)[g:g + n, g:g + n]
+    assert np.all(np.abs(u) + np.abs(v) > 0)
+    # wall-normal velocities of the first and last real cells are O(dx), tangent to the walls
+    assert np.max(np.abs(u[:, 0])) < 0.5 * np.pi / n and np.max(np.abs(v[0, :])) < 0.5 * np.pi / n
+    rho = a["rho"].reshape(-1, rl)[g:g + n, g:g + n]
+    e = a["Emat"].reshape(-1, rl)[g:g + n, g:g + n] - 0.5 * (u * u + v * v)
+    assert np.allclose(0.4 * rho * e, np.where(rho == 1.0, 1.0, 0.1), rtol=1e-14)

@@ -87,10 +87,10 @@ def test_batch_matches_oracle_per_domain(armon, po, oracle):
 
 
 @pytest.mark.parametrize("tuning", [(16, 0, 0, 128), (64, 7, 5, 512), (1000, 1, 1, 256), (33, 300, 0, 128),
-                                    (8, 3, 0, 512)])
+                                    (8, 3, 0, 512), (48, 0, 2, 256, 2), (128, 0, 0, 512, 2)])
 def test_batch_launch_configurations(armon, po, oracle, tuning):
-    """Y run lengths, X tile widths and persistent grids change the work split, never the result
-    (tuning = (Y run rows, X grid, Y grid, X tile width))."""
+    """Y run lengths, X tile widths, cells per thread and persistent grids change the work split,
+    never the result (tuning = (Y run rows, X grid, Y grid, X tile cells[, X cells per thread]))."""
     specs = small_specs(armon, 5, 6)
     print(f"case batch seed 5 tuning {tuning}", flush=True)
     b = run_batch(armon, specs, tuning=tuning, **dict(BASE, max_cycles=25))
@@ -104,7 +104,8 @@ def test_batch_autotune(armon, po, oracle):
     b = run_batch(armon, specs, autotune=True, **dict(BASE, max_cycles=20))
     t = b.tuning()
     assert t.y_run_rows in (16, 32, 64, 128, 256, 512, 1024) and t.ms_x > 0 and t.ms_y > 0
-    assert t.tile_x in (128, 256, 512)
+    assert t.tile_x in (128, 256, 512) and t.x_cells_per_thread in (1, 2)
+    assert t.x_cells_per_thread == 1 or (t.tile_x in (256, 512) and t.grid_x == 0)
     assert t.items_x > 0 and t.items_y > 0
     compare_batch(armon, po, oracle, b, specs)
     b.close()
@@ -221,3 +222,25 @@ def test_batch_non_unit_boundary_factors(armon, po, oracle):
     b = run_batch(armon, specs, bc_factors=factors, **BASE)
     compare_batch(armon, po, oracle, b, specs)
     b.close()
+
+
+def test_batch_set_tuning_rejects_invalid_cells_per_thread(armon):
+    """Two cells per thread exist for X tiles of 256 and 512 cells with one block per tile only;
+    other values are refused with the tuning left as it was."""
+    specs = small_specs(armon, 5, 3)
+    b = armon.Batch(armon.batch_params(**dict(BASE, max_cycles=5)), specs)
+    try:
+        b.init()
+        before = b.tuning()
+        for bad in ((16, 0, 0, 128, 2), (16, 4, 0, 256, 2), (16, 0, 0, 256, 3), (16, 0, 0, 256, -1)):
+            with pytest.raises(armon.ArmonError, match="cells per thread|x_cells_per_thread"):
+                b.set_tuning(*bad)
+            armon.errors()
+            t = b.tuning()
+            assert (t.tile_x, t.x_cells_per_thread, t.grid_x, t.y_run_rows) == \
+                (before.tile_x, before.x_cells_per_thread, before.grid_x, before.y_run_rows)
+        b.set_tuning(16, 0, 0, 512, 2)
+        t = b.tuning()
+        assert (t.tile_x, t.x_cells_per_thread, t.y_run_rows) == (512, 2, 16)
+    finally:
+        b.close()

@@ -35,7 +35,8 @@ with open(os.path.join(HERE, "golden", "full_runs.json")) as _f:
 RUNS = [("sedov_1024_to_end_time", 1), ("sedov_1024_to_end_time", 0),
         ("bizarrium_4096_superbee_500_steps", 1), ("sod_circ_8192_200_steps", 1),
         ("sedov_corner_1024_to_end_time", 1), ("sedov_corner_1024_to_end_time", 0),
-        ("sod_circ_r02_8192_200_steps", 1), ("bizarrium_left_moving_4096_superbee_500_steps", 1)]
+        ("sod_circ_r02_8192_200_steps", 1), ("sod_circ_r02_swirl_8192_200_steps", 1),
+        ("bizarrium_left_moving_4096_superbee_500_steps", 1)]
 
 
 def state_cells(s, f):
