@@ -41,7 +41,7 @@
 namespace armon {
 namespace sweep {
 
-constexpr int kTile = 256;    // X sweep: threads (cells) per tile of the single-domain sweep
+constexpr int kTile = 256;    // X sweep: cells per tile (FP64 single-domain sweep, batched default)
 constexpr int kYBlock = 256;  // Y sweep: columns per block
 
 // Y march: rows per unrolled loop body, the minimum waves per SIMD the register allocation
@@ -410,16 +410,30 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
     if (a.reduce) reduce_cfl(a.st, local);
 }
 
+// Tile of the single-domain X sweep per element type: FP64 tiles of 256 cells at one cell per
+// thread; FP32 tiles of 512 cells at two cells per thread (256 threads, the same 20 KiB of LDS),
+// which measured faster for the FP32-mixed sweep and slower for the FP64 one, whose 3 VALU
+// instructions of 4 are FP64 arithmetic with little latency left to hide.
+template <class T>
+struct XTile {
+    static constexpr int cells = 256, cpt = 1;
+};
+template <>
+struct XTile<float> {
+    static constexpr int cells = 512, cpt = 2;
+};
+
 template <class T, int FLUX, int PROJ, int EOS, int DXM>
-__global__ void __launch_bounds__(kTile) k_sweep_x(SweepArgs<T> a)
+__global__ void __launch_bounds__(XTile<T>::cells / XTile<T>::cpt) k_sweep_x(SweepArgs<T> a)
 {
-    constexpr int W = kTile - 2 * (2 + (FLUX > 0 ? 1 : 0) + (PROJ == 2 ? 1 : 0));
+    constexpr int TILE = XTile<T>::cells;
+    constexpr int W = TILE - 2 * (2 + (FLUX > 0 ? 1 : 0) + (PROJ == 2 ? 1 : 0));
     if (!a.st->active) return;
     const T dt = (T) (a.st->current_dt * a.dt_factor);
-    __shared__ SmemX<T> sm;
+    __shared__ SmemX<T, TILE> sm;
     const int s0 = xcd_block(blockIdx.x, gridDim.x) * W;   // first slot of the tile
     const int j0 = s0 / a.x_slots;
-    sweep_x_tile<T, FLUX, PROJ, EOS, DXM>(a, dt, j0, s0 - j0 * a.x_slots, sm);
+    sweep_x_tile<T, FLUX, PROJ, EOS, DXM, TILE, XTile<T>::cpt>(a, dt, j0, s0 - j0 * a.x_slots, sm);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -793,12 +807,12 @@ void launch_variant(int axis, const SweepArgs<T>& a, hipStream_t s)
 {
     constexpr int R = 2 + (F > 0 ? 1 : 0) + (P == 2 ? 1 : 0);
     if (axis == 0) {
-        constexpr int W = kTile - 2 * R;
+        constexpr int W = XTile<T>::cells - 2 * R;
         SweepArgs<T> b = a;
         b.x_slots = a.nx + 2 * R;
         b.x_inv_slots = 1.0f / (float) b.x_slots;
         const int64_t tiles = x_tiles((int64_t) a.ny * b.x_slots, R, W);
-        k_sweep_x<T, F, P, E, DXM><<<dim3((unsigned) tiles), dim3(kTile), 0, s>>>(b);
+        k_sweep_x<T, F, P, E, DXM><<<dim3((unsigned) tiles), dim3(XTile<T>::cells / XTile<T>::cpt), 0, s>>>(b);
         check_launch("sweep_x");
     } else {
         // Each block marches down a run of rows of its 256 columns (the warm-up is paid once
