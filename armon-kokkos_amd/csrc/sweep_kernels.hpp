@@ -448,45 +448,48 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
     w[N - 1] = v;
 }
 
-// One Y run: column x (lane_ok false for lanes past the domain), output rows [y0, y1),
-// with the cycle's dt. The whole wave must call it with the same y0, y1. AHEAD: load each
-// row's state one row before it is used (the FP64 sweeps); with STALE = false, when the
-// stencil width covers the read extent so that no ghost layer is stale, the march then loads
-// nothing else.
+// The march state of one column of a Y run: sliding windows of the stage results along y.
+// AHEAD: each row's state is loaded one row before it is used (the FP64 sweeps); with
+// STALE = false, when the stencil width covers the read extent so that no ghost layer is stale,
+// the march then loads nothing else.
 template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD>
-__device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, bool lane_ok, int y0, int y1)
-{
-    constexpr int GAD = FLUX > 0 ? 1 : 0;
-    constexpr int E2 = PROJ == 2 ? 1 : 0;
-    constexpr int R = 2 + GAD + E2;
-    const T dx = (T) a.dx;
-    const T inv_dx = (T) a.inv_dx;
+struct YColumn {
+    static constexpr int GAD = FLUX > 0 ? 1 : 0;
+    static constexpr int E2 = PROJ == 2 ? 1 : 0;
+    static constexpr int R = 2 + GAD + E2;
 
-    // Windows, oldest first (see the index map in each stage below).
+    // Windows, oldest first (see the index map in each stage of step()).
     Cell<T> pre[3];                       // cells n-2, n-1, n (pre-update state + EOS)
     T g_u[3], g_p[3];                     // Godunov at faces n-2, n-1, n
     T f_u[5], f_p[2];                     // final face flux: u* at faces F-4..F, p* at F-1..F (F = n - GAD)
     T q_rho[4], q_un[4], q_ut[4], q_E[4]; // post-update cells U-3..U (U = F - 1)
     T h0[2], h1[2], h2[2], h3[2];         // advection fluxes at faces A-1, A (A = U - E2)
     T cw[R + 1];                          // sound speed of cells n-R..n
-#pragma unroll
-    for (int i = 0; i < 3; i++) { pre[i] = benign_cell<T>(); g_u[i] = T(0); g_p[i] = T(0); }
-#pragma unroll
-    for (int i = 0; i < 5; i++) f_u[i] = T(0);
-#pragma unroll
-    for (int i = 0; i < 4; i++) { q_rho[i] = T(1); q_un[i] = T(0); q_ut[i] = T(0); q_E[i] = T(1); }
-    f_p[0] = f_p[1] = T(0);
-    h0[0] = h0[1] = h1[0] = h1[1] = h2[0] = h2[1] = h3[0] = h3[1] = T(0);
-#pragma unroll
-    for (int i = 0; i <= R; i++) cw[i] = T(1);
+    T nx_rho, nx_u, nx_v, nx_E;           // the next row's state (AHEAD)
+    int x;
+    bool lane_ok;                         // false for lanes past the domain
+    double local;                         // CFL bound of the column's outputs
 
-    double local = DBL_MAX;
+    __device__ __forceinline__ void init(int x_, bool ok)
+    {
+        x = x_;
+        lane_ok = ok;
+#pragma unroll
+        for (int i = 0; i < 3; i++) { pre[i] = benign_cell<T>(); g_u[i] = T(0); g_p[i] = T(0); }
+#pragma unroll
+        for (int i = 0; i < 5; i++) f_u[i] = T(0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) { q_rho[i] = T(1); q_un[i] = T(0); q_ut[i] = T(0); q_E[i] = T(1); }
+        f_p[0] = f_p[1] = T(0);
+        h0[0] = h0[1] = h1[0] = h1[1] = h2[0] = h2[1] = h3[0] = h3[1] = T(0);
+#pragma unroll
+        for (int i = 0; i <= R; i++) cw[i] = T(1);
+        nx_rho = T(1); nx_u = T(0); nx_v = T(0); nx_E = T(1);
+        local = DBL_MAX;
+    }
 
-    // AHEAD: the state of the next row's cell is loaded one row ahead, so that its memory
-    // latency is covered by a whole row of arithmetic.
-    constexpr bool kAhead = AHEAD;
-    T nx_rho = T(1), nx_u = T(0), nx_v = T(0), nx_E = T(1);
-    auto fetch = [&](int k) {
+    __device__ __forceinline__ void fetch(const SweepArgs<T>& a, int k)
+    {
         if (lane_ok) {
             const int idx = cell_source<T, 1, STALE>(a, x, k, a.ny).idx;
             nx_rho = a.in_rho[idx];
@@ -494,19 +497,19 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
             nx_v = a.in_v[idx];
             nx_E = a.in_E[idx];
         }
-    };
-    if constexpr (kAhead) fetch(y0 - R);
+    }
 
-#pragma unroll kYUnroll
-    for (int n = y0 - R; n < y1 + R; ++n) {
+    // Row n of a run that outputs rows [y0, y1).
+    __device__ __forceinline__ void step(const SweepArgs<T>& a, T dt, T dx, T inv_dx, int n, int y0, int y1)
+    {
         Cell<T> cn;
-        if constexpr (kAhead) {
+        if constexpr (AHEAD) {
             // the next load is issued after this row's cell is made: a stale layer's p, c loads
             // are waited for there
             cn = lane_ok ? make_cell<T, 1, EOS, STALE>(a, cell_source<T, 1, STALE>(a, x, n, a.ny), nx_rho, nx_u, nx_v,
                                                        nx_E)
                          : benign_cell<T>();
-            if (n + 1 < y1 + R) fetch(n + 1);
+            if (n + 1 < y1 + R) fetch(a, n + 1);
         } else {
             cn = lane_ok ? load_cell<T, 1, EOS>(a, x, n, a.ny) : benign_cell<T>();
         }
@@ -590,7 +593,24 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
                 local = kmin(local, cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) v, (double) u, (double) cw[0], 1.0));
         }
     }
-    if (a.reduce) reduce_cfl(a.st, local);
+};
+
+// One Y run: column x (a lane at or past nx marches benign cells and writes nothing), output
+// rows [y0, y1), with the cycle's dt. The whole wave must call it with the same y0, y1.
+template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD>
+__device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, int y0, int y1)
+{
+    using Col = YColumn<T, FLUX, PROJ, EOS, DXM, STALE, AHEAD>;
+    constexpr int R = Col::R;
+    Col col;
+    col.init(x, x < a.nx);
+    if constexpr (AHEAD) col.fetch(a, y0 - R);
+    // a row counter from 0 keeps the loop and the row index in scalar registers (counting
+    // from y0 - R, the compiler carried the row index in a vector register)
+    const int rows = y1 - y0 + 2 * R;
+#pragma unroll kYUnroll
+    for (int i = 0; i < rows; ++i) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + i, y0, y1);
+    if (a.reduce) reduce_cfl(a.st, col.local);
 }
 
 template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE>
@@ -604,7 +624,7 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     if (x - (int) (threadIdx.x & 63) >= a.nx) return;   // a wave past the last column (no barriers follow)
     const int y0 = seg * a.seg_len;
     constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
-    sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead>(a, dt, x, x < a.nx, y0, min(y0 + a.seg_len, a.ny));
+    sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead>(a, dt, x, y0, min(y0 + a.seg_len, a.ny));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -706,7 +726,7 @@ __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)
         const int x = item.x0 + lane;
         constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
         sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
-                                                                       x < a.nx, item.y0, item.y1);
+                                                                       item.y0, item.y1);
     }
 }
 
@@ -726,7 +746,7 @@ __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)
     const int x = item.x0 + lane;
     constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
     sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
-                                                                   x < a.nx, item.y0, item.y1);
+                                                                   item.y0, item.y1);
 }
 
 // X tiles covering a stream of `slots` slots (tiles of W outputs overlapping by 2R): the last
