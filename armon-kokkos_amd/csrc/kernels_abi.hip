@@ -458,6 +458,143 @@ __global__ void __launch_bounds__(kBlock) k_advection_second_order(Iter2D it, in
     });
 }
 
+// advection_second_order (euler_projection.cpp:83-118) reads, for each face i, the face velocities
+// u* at i-2s..i+2s and the state at i-2s..i+s (the donor side's part of them) straight from
+// memory, 18 loads per face. The two kernels below load each element once and keep the
+// neighbours close (same arithmetic, same bits):
+//  - along rows (s = 1), through LDS: a block stages kAdvTile consecutive elements of one row
+//    and writes the kAdvTile - 4 inner faces;
+//  - across rows (s = the row step), in registers: each thread marches down a run of rows of
+//    its column with windows of the last rows.
+// Each loads only elements of the spans that advection_second_order checks.
+constexpr int kAdvTile = 256;
+constexpr int kAdvOut = kAdvTile - 4;
+
+__global__ void __launch_bounds__(kAdvTile) k_advection_second_order_x(Iter2D it, flt_t dx, flt_t dt,
+                                                                      const flt_t* __restrict__ ustar,
+                                                                      const flt_t* __restrict__ rho,
+                                                                      const flt_t* __restrict__ umat,
+                                                                      const flt_t* __restrict__ vmat,
+                                                                      const flt_t* __restrict__ Emat,
+                                                                      flt_t* __restrict__ a_rho,
+                                                                      flt_t* __restrict__ a_urho,
+                                                                      flt_t* __restrict__ a_vrho,
+                                                                      flt_t* __restrict__ a_Erho)
+{
+    __shared__ flt_t s_us[kAdvTile], s_r[kAdvTile], s_u[kAdvTile], s_v[kAdvTile], s_E[kAdvTile];
+    const int t = threadIdx.x;
+    const int64_t col = (int64_t) blockIdx.x * kAdvOut - 2 + t;   // this thread's element
+    for (int64_t row = it.first_row + blockIdx.y; row <= it.last_row; row += gridDim.y) {
+        int64_t lo, hi;
+        row_columns(it, row, lo, hi);
+        const int64_t i = it.base + row * it.step + col;
+        // u* of faces lo-2 .. hi+1, the state of cells lo-2 .. hi
+        if (col >= lo - 2 && col <= hi + 1) s_us[t] = ustar[i];
+        if (col >= lo - 2 && col <= hi) {
+            s_r[t] = rho[i];
+            s_u[t] = umat[i];
+            s_v[t] = vmat[i];
+            s_E[t] = Emat[i];
+        }
+        __syncthreads();
+        if (t >= 2 && t < 2 + kAdvOut && col >= lo && col < hi) {
+            const flt_t disp = dt * s_us[t];
+            flt_t Dx;
+            int d = t;
+            if (disp > 0) {
+                Dx = -(dx - dt * s_us[t - 1]);
+                d = t - 1;
+            } else {
+                Dx = dx + dt * s_us[t + 1];
+            }
+            flt_t f0, f1, f2, f3;
+            advection_flux_2<flt_t>(dx, dt, disp, Dx, s_us[d - 1], s_us[d], s_us[d + 1], s_us[d + 2], s_r[d - 1],
+                                    s_r[d], s_r[d + 1], s_u[d - 1], s_u[d], s_u[d + 1], s_v[d - 1], s_v[d], s_v[d + 1],
+                                    s_E[d - 1], s_E[d], s_E[d + 1], f0, f1, f2, f3);
+            a_rho[i] = f0;
+            a_urho[i] = f1;
+            a_vrho[i] = f2;
+            a_Erho[i] = f3;
+        }
+        __syncthreads();   // the next row reuses the LDS
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_advection_second_order_y(Iter2D it, int64_t run, flt_t dx, flt_t dt,
+                                                                    const flt_t* __restrict__ ustar,
+                                                                    const flt_t* __restrict__ rho,
+                                                                    const flt_t* __restrict__ umat,
+                                                                    const flt_t* __restrict__ vmat,
+                                                                    const flt_t* __restrict__ Emat,
+                                                                    flt_t* __restrict__ a_rho,
+                                                                    flt_t* __restrict__ a_urho,
+                                                                    flt_t* __restrict__ a_vrho,
+                                                                    flt_t* __restrict__ a_Erho)
+{
+    const int64_t col = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= it.len) return;
+    // rows of this column in the range: [r_lo, r_hi]
+    const int64_t r_lo = it.first_row + (col < it.start - it.first_row * it.len ? 1 : 0);
+    const int64_t r_hi = it.last_row - (col > it.end - 1 - it.last_row * it.len ? 1 : 0);
+    const int64_t s = it.step;
+    const int64_t nruns = (it.last_row - it.first_row + run) / run;
+    for (int64_t k = blockIdx.y; k < nruns; k += gridDim.y) {
+        const int64_t ra = max(it.first_row + k * run, r_lo);
+        const int64_t rb = min(it.first_row + (k + 1) * run - 1, r_hi);
+        if (ra > rb) continue;
+        int64_t i = it.base + ra * s + col;
+        // windows at face/cell row `row`: u* of rows row-2 .. row+2, the state of rows row-2 .. row+1
+        flt_t w_us[5], w_r[4], w_u[4], w_v[4], w_E[4];
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) {
+            const int64_t j = i + (k2 - 2) * s;
+            w_us[k2] = ustar[j];
+            w_r[k2] = rho[j];
+            w_u[k2] = umat[j];
+            w_v[k2] = vmat[j];
+            w_E[k2] = Emat[j];
+        }
+        w_us[4] = ustar[i + 2 * s];
+        for (int64_t row = ra; row <= rb; row++, i += s) {
+            const flt_t disp = dt * w_us[2];
+            flt_t f0, f1, f2, f3;
+            if (disp > 0) {
+                const flt_t Dx = -(dx - dt * w_us[1]);
+                // donor row-1: u* of rows row-2 .. row+1, the state of rows row-2 .. row
+                advection_flux_2<flt_t>(dx, dt, disp, Dx, w_us[0], w_us[1], w_us[2], w_us[3], w_r[0], w_r[1], w_r[2],
+                                        w_u[0], w_u[1], w_u[2], w_v[0], w_v[1], w_v[2], w_E[0], w_E[1], w_E[2], f0, f1,
+                                        f2, f3);
+            } else {
+                const flt_t Dx = dx + dt * w_us[3];
+                // donor row: u* of rows row-1 .. row+2, the state of rows row-1 .. row+1
+                advection_flux_2<flt_t>(dx, dt, disp, Dx, w_us[1], w_us[2], w_us[3], w_us[4], w_r[1], w_r[2], w_r[3],
+                                        w_u[1], w_u[2], w_u[3], w_v[1], w_v[2], w_v[3], w_E[1], w_E[2], w_E[3], f0, f1,
+                                        f2, f3);
+            }
+            a_rho[i] = f0;
+            a_urho[i] = f1;
+            a_vrho[i] = f2;
+            a_Erho[i] = f3;
+            if (row < rb) {
+#pragma unroll
+                for (int k2 = 0; k2 < 4; k2++) w_us[k2] = w_us[k2 + 1];
+#pragma unroll
+                for (int k2 = 0; k2 < 3; k2++) {
+                    w_r[k2] = w_r[k2 + 1];
+                    w_u[k2] = w_u[k2 + 1];
+                    w_v[k2] = w_v[k2 + 1];
+                    w_E[k2] = w_E[k2 + 1];
+                }
+                w_us[4] = ustar[i + 3 * s];
+                w_r[3] = rho[i + 2 * s];
+                w_u[3] = umat[i + 2 * s];
+                w_v[3] = vmat[i + 2 * s];
+                w_E[3] = Emat[i + 2 * s];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_boundary_conditions(Iter1D it, int64_t stencil_width, int64_t disp,
                                                                 flt_t u_factor, flt_t v_factor, flt_t* rho,
                                                                 flt_t* umat, flt_t* vmat, flt_t* pmat, flt_t* cmat,
@@ -966,10 +1103,31 @@ ARMON_KERNEL_TRY
     check_view(advection_vrho, "advection_vrho", sp, false);
     check_view(advection_Erho, "advection_Erho", sp, false);
     if (empty_2d(it)) return;
-    k_advection_second_order<<<grid_2d(it), kBlock, 0, stream()>>>(it, s, dx, dt, ustar.data, rho.data, umat.data,
-                                                                    vmat.data, Emat.data, advection_rho.data,
-                                                                    advection_urho.data, advection_vrho.data,
-                                                                    advection_Erho.data);
+    const int64_t rows = it.last_row - it.first_row + 1;
+    if (s == 1) {
+        // one block per kAdvOut faces of a row
+        const dim3 g((unsigned) ((it.len + kAdvOut - 1) / kAdvOut), (unsigned) (rows < 65535 ? rows : 65535));
+        k_advection_second_order_x<<<g, kAdvTile, 0, stream()>>>(it, dx, dt, ustar.data, rho.data, umat.data,
+                                                                 vmat.data, Emat.data, advection_rho.data,
+                                                                 advection_urho.data, advection_vrho.data,
+                                                                 advection_Erho.data);
+    } else if (s == it.step) {
+        // runs of up to 64 rows, shorter while that leaves fewer than 8192 waves of 64 columns
+        int64_t run = 64;
+        const int64_t groups = (it.len + 63) / 64;
+        while (run > 8 && groups * ((rows + run - 1) / run) < 8192) run /= 2;
+        const int64_t nruns = (rows + run - 1) / run;
+        const dim3 g((unsigned) ((it.len + kBlock - 1) / kBlock), (unsigned) (nruns < 65535 ? nruns : 65535));
+        k_advection_second_order_y<<<g, kBlock, 0, stream()>>>(it, run, dx, dt, ustar.data, rho.data, umat.data,
+                                                               vmat.data, Emat.data, advection_rho.data,
+                                                               advection_urho.data, advection_vrho.data,
+                                                               advection_Erho.data);
+    } else {
+        k_advection_second_order<<<grid_2d(it), kBlock, 0, stream()>>>(it, s, dx, dt, ustar.data, rho.data,
+                                                                        umat.data, vmat.data, Emat.data,
+                                                                        advection_rho.data, advection_urho.data,
+                                                                        advection_vrho.data, advection_Erho.data);
+    }
     after_launch("advection_second_order");
 }
 ARMON_KERNEL_CATCH("advection_second_order")

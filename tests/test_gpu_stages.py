@@ -174,6 +174,42 @@ ADV = ["ustar", "rho", "umat", "vmat", "Emat", "work_array_1", "work_array_2", "
 
 
 @pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("cut", ["full", "partial"])
+def test_advection_second_order_tiles_and_runs(armon, po, oracle, axis, cut):
+    """The second-order advection kernels that load each element once (LDS tiles of 252 faces
+    along x, runs of rows along y): a domain of several tiles and runs, over the whole advection
+    range or a range cut in the middle of its first and last rows, with velocities of both signs
+    (both donor sides), bit for bit against the oracle."""
+    lib = armon.load()
+    p = domain(armon, axis=axis, proj=2, nx=600, ny=300)
+    c = Case(armon, p.nb_cells, ADV, seed=SEED + 31 + axis)
+    r, ir, orr, oir = it2d(armon, po, lib.armon_domain_advection(C.byref(p)))
+    if cut == "partial":
+        n = ir.row_range_length
+        r.start, r.end = n // 3 + 5 * n, r.end - n // 2 - 3 * n - 7
+        orr.start, orr.end = r.start, r.end
+    print(f"case advection_second_order axis {axis} range {r.start}..{r.end} of rows of {ir.row_range_length}",
+          flush=True)
+    dt = 0.0071
+    lib.advection_second_order(C.byref(r), C.byref(ir), p.s, p.dx, dt, *[c.v(k) for k in ADV])
+    oracle.o_advection_second_order(C.byref(orr), C.byref(oir), p.s, p.dx, dt, *[c.h(k) for k in ADV])
+    c.compare(ADV[5:])
+
+
+def test_advection_second_order_other_stride(armon, po, oracle):
+    """A stride that is neither 1 nor the row step goes through the generic per-face kernel:
+    s = 2 over the x advection range, bit for bit."""
+    lib = armon.load()
+    p = domain(armon, axis=0, proj=2, nx=60, ny=20)
+    c = Case(armon, p.nb_cells, ADV, seed=SEED + 41)
+    r, ir, orr, oir = it2d(armon, po, lib.armon_domain_advection(C.byref(p)))
+    print("case advection_second_order s=2", flush=True)
+    lib.advection_second_order(C.byref(r), C.byref(ir), 2, p.dx, 0.0071, *[c.v(k) for k in ADV])
+    oracle.o_advection_second_order(C.byref(orr), C.byref(oir), 2, p.dx, 0.0071, *[c.h(k) for k in ADV])
+    c.compare(ADV[5:])
+
+
+@pytest.mark.parametrize("axis", [0, 1])
 def test_advection_first_order(armon, po, oracle, axis):
     lib = armon.load()
     p = domain(armon, axis=axis, proj=1)
