@@ -262,27 +262,35 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
     int slot[CPT], jc[CPT], kc[CPT];
     bool in_range[CPT];
     Cell<T> cell[CPT];
+    // Row and cell of each slot: n = k0 + t < x_slots + TILE. Rows of at least TILE slots:
+    // n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and a
+    // single correction makes it exact. (The test on the row length is the block's, taken
+    // once, so that only one of the two ways is executed.)
+    if (a.x_slots >= TILE) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int n = k0 + (int) threadIdx.x + c * NT;
+            const int dj = n >= a.x_slots ? 1 : 0;
+            jc[c] = j0 + dj;
+            kc[c] = n - (dj ? a.x_slots : 0) - R;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int n = k0 + (int) threadIdx.x + c * NT;
+            int dj = (int) ((float) n * a.x_inv_slots);
+            int r = n - dj * a.x_slots;
+            if (r < 0) { dj--; r += a.x_slots; }
+            else if (r >= a.x_slots) { dj++; r -= a.x_slots; }
+            jc[c] = j0 + dj;
+            kc[c] = r - R;
+        }
+    }
     // Stage 0: load (or synthesise) the cells, EOS.
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int t = threadIdx.x + c * NT;
-        slot[c] = t;
-        // row and cell of the slot: n = k0 + t < x_slots + TILE. Rows of at least TILE slots:
-        // n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and a
-        // single correction makes it exact.
-        const int n = k0 + t;
-        int dj, r;
-        if (a.x_slots >= TILE) {
-            dj = n >= a.x_slots ? 1 : 0;
-            r = n - (dj ? a.x_slots : 0);
-        } else {
-            dj = (int) ((float) n * a.x_inv_slots);
-            r = n - dj * a.x_slots;
-            if (r < 0) { dj--; r += a.x_slots; }
-            else if (r >= a.x_slots) { dj++; r -= a.x_slots; }
-        }
-        jc[c] = j0 + dj;
-        kc[c] = r - R;                    // -R <= k < nx + R
+        slot[c] = t;                      // kc: -R <= k < nx + R
         in_range[c] = jc[c] < a.ny;       // slots past the last row
         cell[c] = in_range[c] ? load_cell<T, 0, EOS>(a, jc[c], kc[c], a.nx) : benign_cell<T>();
         sm.rho[t] = cell[c].rho;
