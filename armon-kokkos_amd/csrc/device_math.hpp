@@ -43,7 +43,7 @@ ARMON_HD double cr_pow4(double y)
 }
 
 // Division. FP64 (and FP32 outside the FP32-mixed kernels) is the IEEE quotient a / b. The
-// FP32-mixed translation unit, which is judged by accuracy rather than bit parity, uses the
+// FP32-mixed translation unit, which is held to an accuracy bound rather than bit parity, uses the
 // native reciprocal instead (about 1 ulp).
 template <class T> ARMON_HD T dv(T a, T b) { return a / b; }
 #if defined(ARMON_FAST_FP32) && defined(__HIP_DEVICE_COMPILE__)

@@ -1,7 +1,7 @@
 // Fused engine, FP32-mixed instantiation of the sweep kernels (sweep_kernels.hpp).
 //
 // Compiled with FMA contraction within each source expression and with the native-rate FP32
-// reciprocal and square root: this form is judged by its L1/Linf error against the FP64
+// reciprocal and square root: this form is held to its L1/Linf error against the FP64
 // engine, not by bit parity; its results do not depend on the launch configuration (a cell's
 // arithmetic is the same in every tile, run and unrolled copy of the march). The Bizarrium
 // EOS, the CFL bound and the conservation sums stay in FP64.
