@@ -135,11 +135,11 @@ __device__ __forceinline__ Cell<T> load_cell(const SweepArgs<T>& a, int q, int k
     return cell;
 }
 
-// load_cell in two parts, for the FP64 Y march, which loads a row's state one row before it
-// makes the cell: where cell `k` comes from (cell_source: the index it is read at, the mirror
-// side or staleness), then the cell from the loaded state (make_cell). The FP32-mixed march
-// keeps load_cell: at 90 VGPRs it runs 5 waves per SIMD, which already cover the load latency
-// (loading a row ahead gained about 1% there).
+// load_cell in two parts, for the Y march, which loads a row's state one row before it makes
+// the cell: where cell `k` comes from (cell_source: the index it is read at, the mirror side or
+// staleness), then the cell from the loaded state (make_cell). The row-ahead loads make the
+// FP64 Y sweep 4-9% faster and the FP32-mixed one 2-3% (its 5 waves per SIMD already cover
+// most of the load latency).
 struct CellSource {
     int idx;
     int mirror;
@@ -631,7 +631,7 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     const int x = (bid - seg * ncolblk) * kYBlock + threadIdx.x;
     if (x - (int) (threadIdx.x & 63) >= a.nx) return;   // a wave past the last column (no barriers follow)
     const int y0 = seg * a.seg_len;
-    constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
+    constexpr bool ahead = ARMON_Y_PREFETCH;
     sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead>(a, dt, x, y0, min(y0 + a.seg_len, a.ny));
 }
 
@@ -732,7 +732,7 @@ __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)
         if (!st->active) continue;   // the same for the whole wave
         const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
         const int x = item.x0 + lane;
-        constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
+        constexpr bool ahead = ARMON_Y_PREFETCH;
         sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
                                                                        item.y0, item.y1);
     }
@@ -752,7 +752,7 @@ __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)
     if (!st->active) return;
     const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
     const int x = item.x0 + lane;
-    constexpr bool ahead = ARMON_Y_PREFETCH && std::is_same_v<T, double>;
+    constexpr bool ahead = ARMON_Y_PREFETCH;
     sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
                                                                    item.y0, item.y1);
 }
