@@ -262,10 +262,10 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
     int slot[CPT], jc[CPT], kc[CPT];
     bool in_range[CPT];
     Cell<T> cell[CPT];
-    // Row and cell of each slot: n = k0 + t < x_slots + TILE. Rows of at least TILE slots:
-    // n < 2 x_slots. Shorter rows: the float quotient is within one of the exact one and a
-    // single correction makes it exact. (The test on the row length is the block's, taken
-    // once, so that only one of the two ways is executed.)
+    // Row jc and cell kc (-R <= kc < nx + R) of each slot: n = k0 + t < x_slots + TILE. Rows of
+    // at least TILE slots: n < 2 x_slots. Shorter rows: the float quotient is within one of the
+    // exact one and a single correction makes it exact. (The test on the row length is the
+    // block's, taken once, so that only one of the two ways is executed.)
     if (a.x_slots >= TILE) {
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -290,8 +290,8 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<T>& a, T dt, int j0
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int t = threadIdx.x + c * NT;
-        slot[c] = t;                      // kc: -R <= k < nx + R
-        in_range[c] = jc[c] < a.ny;       // slots past the last row
+        slot[c] = t;
+        in_range[c] = jc[c] < a.ny;       // false for slots past the last row
         cell[c] = in_range[c] ? load_cell<T, 0, EOS>(a, jc[c], kc[c], a.nx) : benign_cell<T>();
         sm.rho[t] = cell[c].rho;
         sm.un[t] = cell[c].un;
