@@ -242,8 +242,9 @@ static int run_batch(ArmonParams common, const BatchOptions& bo)
         ArmonBatchTuning t;
         if (armon_batch_autotune(b, &t) != 0) goto done;
         if (common.verbose < 3)
-            printf("Autotuned: X tiles of %d cells, Y runs of %d rows, X grid %d, Y grid %d (%.3f + %.3f ms per "
-                   "sweep pair)\n", t.tile_x, t.y_run_rows, t.grid_x, t.grid_y, t.ms_x, t.ms_y);
+            printf("Autotuned: X tiles of %d cells at %d per thread, Y runs of %d rows, X grid %d, Y grid %d "
+                   "(%.3f + %.3f ms per sweep pair)\n", t.tile_x, t.x_cells_per_thread, t.y_run_rows, t.grid_x,
+                   t.grid_y, t.ms_x, t.ms_y);
     }
     {
         if (armon_batch_begin(b) != 0) goto done;
