@@ -935,10 +935,13 @@ ARMON_KERNEL_TRY
             k_acoustic_GAD_x<LL><<<g, kGadTile, 0, stream()>>>(it, dt, dx, ustar.data, pstar.data, rho.data, u.data,
                                                                pmat.data, cmat.data);
         } else if (s == it.step) {
-            // runs of up to 64 rows, shorter while that leaves fewer than 8192 waves of 64 columns
-            int64_t run = 64;
+            // runs of 8 rows (4 while that leaves fewer than 8192 waves of 64 columns): these
+            // kernels wait on memory, and short runs keep more waves in flight; the rows a run
+            // re-reads before its first output come from L2 (measured at 8192^2: 8 rows 12-16%
+            // faster than 64, 4 rows the same as 8)
+            int64_t run = 8;
             const int64_t groups = (it.len + 63) / 64;
-            while (run > 8 && groups * ((rows + run - 1) / run) < 8192) run /= 2;
+            if (groups * ((rows + run - 1) / run) < 8192) run = 4;
             const int64_t nruns = (rows + run - 1) / run;
             const dim3 g((unsigned) ((it.len + kBlock - 1) / kBlock), (unsigned) (nruns < 65535 ? nruns : 65535));
             k_acoustic_GAD_y<LL><<<g, kBlock, 0, stream()>>>(it, run, dt, dx, ustar.data, pstar.data, rho.data,
@@ -1112,10 +1115,13 @@ ARMON_KERNEL_TRY
                                                                  advection_urho.data, advection_vrho.data,
                                                                  advection_Erho.data);
     } else if (s == it.step) {
-        // runs of up to 64 rows, shorter while that leaves fewer than 8192 waves of 64 columns
-        int64_t run = 64;
+        // runs of 8 rows (4 while that leaves fewer than 8192 waves of 64 columns): these
+        // kernels wait on memory, and short runs keep more waves in flight; the rows a run
+        // re-reads before its first output come from L2 (measured at 8192^2: 8 rows 12-16%
+        // faster than 64, 4 rows the same as 8)
+        int64_t run = 8;
         const int64_t groups = (it.len + 63) / 64;
-        while (run > 8 && groups * ((rows + run - 1) / run) < 8192) run /= 2;
+        if (groups * ((rows + run - 1) / run) < 8192) run = 4;
         const int64_t nruns = (rows + run - 1) / run;
         const dim3 g((unsigned) ((it.len + kBlock - 1) / kBlock), (unsigned) (nruns < 65535 ? nruns : 65535));
         k_advection_second_order_y<<<g, kBlock, 0, stream()>>>(it, run, dx, dt, ustar.data, rho.data, umat.data,
