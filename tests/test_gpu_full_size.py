@@ -107,3 +107,27 @@ def test_conservation_8192(armon):
     assert abs(m1.value - m0.value) <= 1e-12 * abs(m0.value)
     assert abs(e1.value - e0.value) <= 1e-12 * abs(e0.value)
     s.close()
+
+
+def test_conservation_32768(armon):
+    """The largest square domain below the 2^31-cell limit of the Params indexing that is a
+    power of two: Sedov 32768^2 (1.07 G cells, 137 GB for the reference's 16 FP64 arrays in
+    HBM), 20 cycles of the fused engine, mass and energy conserved to 1e-12."""
+    print("case Sedov 32768^2 fused, 20 cycles", flush=True)
+    p = armon.make_params("Sedov", nx=32768, ny=32768, max_cycles=20, engine=1, **BENCH)
+    s = armon.Solver(p)
+    try:
+        s.init_test()
+        lib = armon.load()
+        s.begin()
+        m0, e0 = C.c_double(), C.c_double()
+        assert lib.armon_solver_conservation(s.h, C.byref(m0), C.byref(e0)) == 0
+        s.advance(20)
+        r = s.finish()
+        m1, e1 = C.c_double(), C.c_double()
+        assert lib.armon_solver_conservation(s.h, C.byref(m1), C.byref(e1)) == 0
+        assert r.status == 0 and r.cycles == 20, armon.errors()
+        assert abs(m1.value - m0.value) <= 1e-12 * abs(m0.value)
+        assert abs(e1.value - e0.value) <= 1e-12 * abs(e0.value)
+    finally:
+        s.close()
