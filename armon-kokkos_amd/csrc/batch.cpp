@@ -56,13 +56,6 @@ T* device_alloc(size_t count, const char* what)
     return static_cast<T*>(p);
 }
 
-bool pow2(double x)
-{
-    if (!(x > 0) || !std::isfinite(x)) return false;
-    int e;
-    return std::frexp(x, &e) == 0.5;
-}
-
 }  // namespace
 
 struct ArmonBatch {

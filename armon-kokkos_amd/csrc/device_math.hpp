@@ -75,6 +75,81 @@ ARMON_HD bool recip_div_ok(double x)
     return x == 0 || e - (1023u - 900u) <= 1800u;
 }
 
+// Quotients over one divisor (FP64). The compiler divides a by b as
+//   b', a' = v_div_scale(b), v_div_scale(a);  y = rcp(b');
+//   y = fma(y, fma(-b', y, 1), y), twice;  q = a' * y;  q = v_div_fmas(fma(-b', q, a'), y, q);
+//   result = v_div_fixup(q, b, a).
+// For b in [2^-200, 2^200] and a zero or of magnitude in [2^-600, 2^300], v_div_scale returns
+// its operand unchanged and clears the flag (it acts only on a zero operand, a subnormal b, 1/b
+// or a/b, a biased exponent of a at or below 53, or exponents of a and b 768 or more apart),
+// v_div_fmas is then the plain FMA, and v_div_fixup returns the quotient with the sign of
+// a / b (it substitutes a value only for NaN, infinite or zero operands and for exponents of a
+// and b more than 1075 apart). Within those ranges the refined reciprocal y depends on b alone,
+// so quotients that share b compute it once and each costs a product and two FMAs, with the
+// bits of a / b. The remainder is taken as RN(b q - a) and negated in the last FMA: equal to
+// the compiler's RN(a - b q) for a != 0, and for a = +-0 (with b > 0) it makes the result the
+// signed zero of a / b. Any operand outside the ranges sends every quotient of the group back
+// to a / b. The rcp instruction runs at a quarter of the FMA rate, so a shared divisor saves
+// about 9 FMA issue slots per extra quotient.
+#if defined(__HIP_DEVICE_COMPILE__)
+ARMON_HD double refined_recip(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+ARMON_HD double refined_quot(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = __builtin_fma(b, q, -a);
+    return __builtin_fma(-r, y, q);
+}
+#endif
+
+ARMON_HD int shared_divisor_ok(double b) { return (int) (b >= 0x1p-200) & (int) (b <= 0x1p200); }
+
+ARMON_HD int shared_numerator_ok(double a)
+{
+    const double m = __builtin_fabs(a);
+    return ((int) (m >= 0x1p-600) | (int) (a == 0)) & (int) (m <= 0x1p300);
+}
+
+// Runs body(div) where div(a) returns a / b for every quotient over b that the body takes.
+// The body may run twice (once more with IEEE division when an operand leaves the ranges
+// above), so it only assigns its results.
+template <class Body>
+ARMON_HD void with_divisor(double b, Body&& body)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = refined_recip(b);
+    int ok = shared_divisor_ok(b);
+    body([&](double a) {
+        ok &= shared_numerator_ok(a);
+        return refined_quot(a, b, y);
+    });
+    // marked unlikely: the compiler then keeps this block behind a branch that waves whose lanes
+    // are all inside the ranges skip, rather than running both sequences for every lane and
+    // selecting
+    if (__builtin_expect(!ok, 0)) body([&](double a) { return a / b; });
+#else
+    body([&](double a) { return a / b; });
+#endif
+}
+
+// with_divisor for FP64 when SHARED; otherwise (and for the FP32 types) each quotient is
+// divided with dv.
+template <class T, bool SHARED, class Body>
+ARMON_HD void with_divisor_t(T b, Body&& body)
+{
+    if constexpr (sizeof(T) == 8 && SHARED)
+        with_divisor((double) b, static_cast<Body&&>(body));
+    else
+        body([&](T a) { return dv<T>(a, b); });
+}
+
 template <class T> ARMON_HD T sqrt_t(T x);
 template <> ARMON_HD double sqrt_t<double>(double x) { return __builtin_sqrt(x); }
 template <> ARMON_HD float sqrt_t<float>(float x) { return __builtin_sqrtf(x); }
@@ -103,14 +178,17 @@ template <int L, class T> ARMON_HD T limiter(T r)
 }
 
 // acoustic.cpp:10-19: linearised acoustic Riemann solver at the face between
-// the left cell (…_im) and the right cell (…_i).
-template <class T>
+// the left cell (…_im) and the right cell (…_i). SHARED: the two quotients share the divisor's
+// refined reciprocal (with_divisor), else each is divided.
+template <class T, bool SHARED = true>
 ARMON_HD void godunov(T rho_i, T rho_im, T c_i, T c_im, T u_i, T u_im, T p_i, T p_im, T& ustar, T& pstar)
 {
     T rc_l = rho_im * c_im;
     T rc_r = rho_i * c_i;
-    ustar = dv<T>(rc_l * u_im + rc_r * u_i + (p_im - p_i), rc_l + rc_r);
-    pstar = dv<T>(rc_r * p_im + rc_l * p_i + rc_l * rc_r * (u_im - u_i), rc_l + rc_r);
+    with_divisor_t<T, SHARED>(rc_l + rc_r, [&](auto div) {
+        ustar = div(rc_l * u_im + rc_r * u_i + (p_im - p_i));
+        pstar = div(rc_r * p_im + rc_l * p_i + rc_l * rc_r * (u_im - u_i));
+    });
 }
 
 // acoustic.cpp:65-86: GAD second-order correction at face i (between cells i-1 and i),
@@ -164,10 +242,13 @@ ARMON_HD void bizarrium(double rho, double E, double u, double v, double& p_out,
     double x = rho / rho0 - 1;
     double g = G0 * (1 - rho0 / rho);
 
-    double f0 = (1 + (s / 3 - 2) * x + q * (x * x) + r * (x * x * x)) / (1 - s * x);
-    double f1 = (s / 3 - 2 + 2 * q * x + 3 * r * (x * x) + s * f0) / (1 - s * x);
-    double f2 = (2 * q + 6 * r * x + 2 * s * f1) / (1 - s * x);
-    double f3 = (6 * r + 3 * s * f2) / (1 - s * x);
+    double f0, f1, f2, f3;
+    with_divisor(1 - s * x, [&](auto div) {
+        f0 = div(1 + (s / 3 - 2) * x + q * (x * x) + r * (x * x * x));
+        f1 = div(s / 3 - 2 + 2 * q * x + 3 * r * (x * x) + s * f0);
+        f2 = div(2 * q + 6 * r * x + 2 * s * f1);
+        f3 = div(6 * r + 3 * s * f2);
+    });
 
     double eps_k0 = eps0 - Cv0 * T0 * (1 + g) + 0.5 * (K0 / rho0) * (x * x) * f0;
     double pk0 = -Cv0 * T0 * G0 * rho0 + (0.5 * K0 * x * (1 + x) * (1 + x) * (2 * f0 + x * f1));
@@ -254,7 +335,8 @@ ARMON_HD void advection_flux_2(T dx, T dt, T disp, T Dx,
 enum DxMode : int { DX_DIVIDE = 0, DX_POW2 = 1, DX_RECIP = 2 };
 
 // euler_projection.cpp:42-52: remap of cell i from the advection fluxes at faces i, i+1.
-template <int DXM, class T>
+// SHARED: u, v and E share the new density's refined reciprocal (with_divisor).
+template <int DXM, class T, bool SHARED = true>
 ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
                          T f_rho_i, T f_rho_ip, T f_urho_i, T f_urho_ip,
                          T f_vrho_i, T f_vrho_ip, T f_Erho_i, T f_Erho_ip,
@@ -292,9 +374,11 @@ ARMON_HD void projection(T dx, T inv_dx, T dt, T us_i, T us_ip,
         n_Erho = dv<T>(dX * rho * E - (f_Erho_ip - f_Erho_i), dx);
     }
     rho = n_rho;
-    u = dv<T>(n_urho, n_rho);
-    v = dv<T>(n_vrho, n_rho);
-    E = dv<T>(n_Erho, n_rho);
+    with_divisor_t<T, SHARED>(n_rho, [&](auto div) {
+        u = div(n_urho);
+        v = div(n_vrho);
+        E = div(n_Erho);
+    });
 }
 
 // reductions.cpp:18-20: CFL bound of one cell (mask 1 for real cells).

@@ -460,6 +460,14 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
 // AHEAD: each row's state is loaded one row before it is used (the FP64 sweeps); with
 // STALE = false, when the stencil width covers the read extent so that no ghost layer is stale,
 // the march then loads nothing else.
+// Whether the Y march shares divisors between Godunov's two quotients and the projection's three
+// (with_divisor). The X tiles do, which removes 3 of their 17 FP64 reciprocals per cell and 3%
+// of their time; in the march, which keeps 3 waves per SIMD and waits on its dependency chains
+// more than on instruction issue, the same change cut the VALU busy cycles by 4% without
+// shortening the sweep (the range checks and their scalar mask work lengthen the chains), so it
+// divides each quotient.
+constexpr bool kYSharedDivisor = false;
+
 template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD>
 struct YColumn {
     static constexpr int GAD = FLUX > 0 ? 1 : 0;
@@ -528,7 +536,8 @@ struct YColumn {
 
         // Godunov at face n (cells n-1 | n).
         T gu, gp;
-        godunov<T>(pre[2].rho, pre[1].rho, pre[2].c, pre[1].c, pre[2].un, pre[1].un, pre[2].p, pre[1].p, gu, gp);
+        godunov<T, kYSharedDivisor>(pre[2].rho, pre[1].rho, pre[2].c, pre[1].c, pre[2].un, pre[1].un, pre[2].p,
+                                    pre[1].p, gu, gp);
         push(g_u, gu);
         push(g_p, gp);
 
@@ -590,8 +599,8 @@ struct YColumn {
             constexpr int QP = E2 ? 1 : 2;    // q index of cell P
             constexpr int FP = E2 ? 1 : 2;    // f_u index of face P
             T r = q_rho[QP], u = q_un[QP], v = q_ut[QP], e = q_E[QP];
-            projection<DXM, T>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1], h2[0], h2[1], h3[0],
-                               h3[1], r, u, v, e);
+            projection<DXM, T, kYSharedDivisor>(dx, inv_dx, dt, f_u[FP], f_u[FP + 1], h0[0], h0[1], h1[0], h1[1],
+                                                h2[0], h2[1], h3[0], h3[1], r, u, v, e);
             const int idx = a.index_start + P * a.row_length + x;
             a.out_rho[idx] = r;
             a.out_u[idx] = v;   // ut is the x velocity in a Y sweep
