@@ -19,6 +19,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libarmon.so")
+LIB_F32_PATH = os.path.join(_HERE, "lib", "libarmon_f32.so")
 CLI_PATH = os.path.join(_HERE, "bin", "armon_hip_2D")
 
 # ---- enumerations (same integer values as the reference) -------------------------------
@@ -253,6 +254,47 @@ def load(path: Optional[str] = None):
     return lib
 
 
+# The float build (flt_t = float, the reference's USE_SINGLE_PRECISION): the kernel table with
+# every flt_t argument and result as float, and the memory plumbing (no flt_t).
+_PD, _PF = C.POINTER(C.c_double), C.POINTER(C.c_float)
+KERNEL_SYMBOLS_F32 = {
+    name: (C.c_float if res is _d else res, [C.c_float if a is _d else _PF if a is _PD else a for a in args])
+    for name, (res, args) in KERNEL_SYMBOLS.items()
+}
+PLUMBING_SYMBOLS = {k: EXTENSION_SYMBOLS[k] for k in (
+    "armon_set_stream", "armon_get_stream", "armon_device_count", "armon_malloc", "armon_free",
+    "armon_memcpy_htod", "armon_memcpy_dtoh", "armon_memcpy_dtod", "armon_synchronize", "armon_last_error",
+    "armon_clear_error", "armon_version")}
+
+_lib_f32 = None
+_handler_ref_f32 = None
+
+
+def load_f32(path: Optional[str] = None):
+    """Loads ``lib/libarmon_f32.so``, the kernel C-ABI built with flt_t = float (common.h:8-12),
+    with the same error recording as :func:`load`. Raises if the library is missing."""
+    global _lib_f32, _handler_ref_f32
+    if _lib_f32 is not None:
+        return _lib_f32
+    path = path or LIB_F32_PATH
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"Armon HIP float library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for table in (KERNEL_SYMBOLS_F32, PLUMBING_SYMBOLS):
+        for name, (res, args) in table.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+
+    def _handler(kernel, msg):
+        _errors.append((kernel.decode(), msg.decode()))
+
+    _handler_ref_f32 = HANDLER_T(_handler)
+    C.c_void_p.in_dll(lib, "raise_exception_handler").value = C.cast(_handler_ref_f32, C.c_void_p).value
+    _lib_f32 = lib
+    return lib
+
+
 def errors(clear: bool = True) -> list:
     """Errors reported through raise_exception_handler since the last call."""
     out = list(_errors)
@@ -273,12 +315,17 @@ def _check(status, what):
 
 # ---- device memory ---------------------------------------------------------------------
 class DeviceArray:
-    """A device buffer of float64 owned by Python, exposed to the kernels as a View."""
+    """A device buffer of flt_t owned by Python, exposed to the kernels as a View: float64 for
+    ``lib/libarmon.so``, float32 with ``single=True`` for ``lib/libarmon_f32.so`` (allocated,
+    copied and synchronised through that library, on its stream)."""
 
-    def __init__(self, n: int, label: Optional[str] = None, host: Optional[np.ndarray] = None):
-        lib = load()
+    def __init__(self, n: int, label: Optional[str] = None, host: Optional[np.ndarray] = None,
+                 single: bool = False):
+        self._load = load_f32 if single else load
+        self.dtype = np.float32 if single else np.float64
+        lib = self._load()
         self.n = int(n)
-        self.ptr = lib.armon_malloc(max(self.n, 1) * 8)
+        self.ptr = lib.armon_malloc(max(self.n, 1) * np.dtype(self.dtype).itemsize)
         if not self.ptr:
             raise MemoryError(lib.armon_last_error().decode())
         self._label = label.encode() if label else None
@@ -290,19 +337,19 @@ class DeviceArray:
         return View(self.ptr, self.n, self._label)
 
     def upload(self, host: np.ndarray):
-        a = np.ascontiguousarray(host, dtype=np.float64)
+        a = np.ascontiguousarray(host, dtype=self.dtype)
         assert a.size == self.n
-        _check(load().armon_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "armon_memcpy_htod")
+        _check(self._load().armon_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "armon_memcpy_htod")
 
     def download(self) -> np.ndarray:
-        out = np.empty(self.n, dtype=np.float64)
-        _check(load().armon_synchronize(), "armon_synchronize")
-        _check(load().armon_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "armon_memcpy_dtoh")
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(self._load().armon_synchronize(), "armon_synchronize")
+        _check(self._load().armon_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "armon_memcpy_dtoh")
         return out
 
     def free(self):
         if self.ptr:
-            load().armon_free(self.ptr)
+            self._load().armon_free(self.ptr)
             self.ptr = None
 
     def __del__(self):

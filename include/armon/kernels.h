@@ -43,14 +43,16 @@ extern "C" {
 #define ARMON_REF(T) T*
 #endif
 
-/* common.h:8-12. The library is built in FP64 only: the reference's single-precision build
- * (USE_SINGLE_PRECISION) has no counterpart here, so a caller asking for it would bind float
- * declarations to double implementations. The reduced-precision form of this library is the
- * fused engine's FP32-mixed precision (armon.h). */
+/* common.h:8-12: flt_t is double, or float in the reference's single-precision build
+ * (USE_SINGLE_PRECISION). The two builds are two libraries: lib/libarmon.so (FP64, with the
+ * solver extensions of armon.h) and lib/libarmon_f32.so (flt_t = float: the kernel entry points
+ * of this header and the memory plumbing of armon.h, no solver). A caller that defines
+ * ARMON_SINGLE_PRECISION links libarmon_f32.so; flt_size() reports which one is loaded. */
 #if defined(ARMON_SINGLE_PRECISION) && ARMON_SINGLE_PRECISION
-#error "libarmon is built in FP64 only: ARMON_SINGLE_PRECISION (the reference's USE_SINGLE_PRECISION) is not supported"
-#endif
+typedef float flt_t;
+#else
 typedef double flt_t;
+#endif
 
 /* indexing.h:14-19 */
 typedef int64_t Idx;

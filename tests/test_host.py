@@ -40,6 +40,17 @@ def test_library_exports_every_declared_symbol(armon):
     assert set(declared_symbols()) <= known
 
 
+def test_float_library_exports_the_kernel_abi(armon):
+    """lib/libarmon_f32.so (flt_t = float): every kernels.h symbol, the plumbing, no solver."""
+    lib = armon.load_f32()
+    kernel_names = [n for n in declared_symbols() if n in armon.KERNEL_SYMBOLS or n == "raise_exception_handler"]
+    assert len(kernel_names) == len(armon.KERNEL_SYMBOLS) + 1
+    missing = [n for n in kernel_names + list(armon.PLUMBING_SYMBOLS) if not hasattr(lib, n)]
+    assert not missing
+    assert not hasattr(lib, "armon_solver_create")
+    assert lib.flt_size() == 4 and lib.idx_size() == 8 and lib.is_uidx_signed() is False
+
+
 def test_abi_queries(armon):
     lib = armon.load()
     assert lib.idx_size() == 8
@@ -256,19 +267,22 @@ def test_fused_engine_refuses_ghost_output(armon):
         assert lib.armon_params_check(C.byref(p)) == 1
 
 
-def test_single_precision_switch_is_refused(tmp_path):
-    """The library is FP64 only: including kernels.h with ARMON_SINGLE_PRECISION set (the
-    reference's USE_SINGLE_PRECISION, common.h:8-12) is a compile error instead of float
-    declarations bound to double implementations; without it flt_t is double."""
+def test_single_precision_switch_selects_the_float_library(tmp_path):
+    """ARMON_SINGLE_PRECISION (the reference's USE_SINGLE_PRECISION, common.h:8-12) makes flt_t
+    float; a C caller built that way and linked to lib/libarmon_f32.so, or built without it and
+    linked to lib/libarmon.so, sees its own flt_t size in flt_size()."""
     import subprocess
 
     src = tmp_path / "use.c"
-    src.write_text('#include "armon/kernels.h"\n'
-                   '_Static_assert(sizeof(flt_t) == 8, "flt_t is double");\nint main(void) { return 0; }\n')
+    src.write_text('#include <stdio.h>\n#include "armon/kernels.h"\n'
+                   'int main(void) { printf("%d %d\\n", (int) sizeof(flt_t), flt_size());'
+                   ' return flt_size() == (int) sizeof(flt_t) ? 0 : 1; }\n')
     inc = os.path.join(ROOT, "include")
-    ok = subprocess.run(["gcc", "-std=c11", "-I", inc, "-c", str(src), "-o", str(tmp_path / "a.o")],
-                        capture_output=True, text=True)
-    assert ok.returncode == 0, ok.stderr
-    bad = subprocess.run(["gcc", "-std=c11", "-DARMON_SINGLE_PRECISION=1", "-I", inc, "-c", str(src), "-o",
-                          str(tmp_path / "b.o")], capture_output=True, text=True)
-    assert bad.returncode != 0 and "FP64 only" in bad.stderr
+    libdir = os.path.join(ROOT, "armon-kokkos_amd", "lib")
+    for flags, lib, size in (([], "armon", 8), (["-DARMON_SINGLE_PRECISION=1"], "armon_f32", 4)):
+        exe = tmp_path / f"use_{lib}"
+        cc = subprocess.run(["gcc", "-std=c11", *flags, "-I", inc, str(src), "-o", str(exe), "-L", libdir,
+                             f"-l{lib}", f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+        assert cc.returncode == 0, cc.stderr
+        run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+        assert run.returncode == 0 and run.stdout.split() == [str(size), str(size)], run.stdout + run.stderr
