@@ -234,6 +234,27 @@ def test_out_of_bounds_reports(armon):
     assert len(errs) == 1 and "out of bounds" in errs[0][1] and "[-1," in errs[0][1]
 
 
+def test_float_library_error_paths(armon):
+    """The float build validates on the host like the FP64 one and reports through its own
+    raise_exception_handler: an invalid limiter, a wrong view order, an out-of-bounds range."""
+    lib = armon.load_f32()
+    armon.errors()
+    names = ["ustar", "pstar", "rho", "umat", "pmat", "cmat"]
+    v, keep = _views(armon, 100, names)
+    r, ir = armon.Range(0, 10), armon.InnerRange2D(20, 10, 0, 10)
+    lib.acoustic_GAD(C.byref(r), C.byref(ir), 1, 0.1, 0.1, *[C.byref(v[k]) for k in names], 7)
+    errs = armon.errors()
+    assert len(errs) == 1 and errs[0][0] == "acoustic_GAD" and "Invalid limiter index: 7" in errs[0][1]
+    lib.acoustic(C.byref(r), C.byref(ir), 1, C.byref(v["pstar"]), C.byref(v["ustar"]),
+                 *[C.byref(v[k]) for k in names[2:]])
+    errs = armon.errors()
+    assert len(errs) == 1 and "Wrong view order" in errs[0][1]
+    v, keep = _views(armon, 29, names)
+    lib.acoustic(C.byref(r), C.byref(ir), 1, *[C.byref(v[k]) for k in names])
+    errs = armon.errors()
+    assert len(errs) == 1 and "out of bounds" in errs[0][1]
+
+
 def test_boundary_conditions_bounds(armon):
     lib = armon.load()
     armon.errors()
