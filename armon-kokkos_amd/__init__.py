@@ -307,9 +307,10 @@ class ArmonError(RuntimeError):
     pass
 
 
-def _check(status, what):
+def _check(status, what, lib=None):
     if status:
-        msg = _lib.armon_last_error().decode() if _lib else ""
+        lib = lib or _lib
+        msg = lib.armon_last_error().decode() if lib else ""
         raise ArmonError(f"{what} failed ({status}): {msg} {errors()}")
 
 
@@ -339,12 +340,14 @@ class DeviceArray:
     def upload(self, host: np.ndarray):
         a = np.ascontiguousarray(host, dtype=self.dtype)
         assert a.size == self.n
-        _check(self._load().armon_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "armon_memcpy_htod")
+        lib = self._load()
+        _check(lib.armon_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "armon_memcpy_htod", lib)
 
     def download(self) -> np.ndarray:
         out = np.empty(self.n, dtype=self.dtype)
-        _check(self._load().armon_synchronize(), "armon_synchronize")
-        _check(self._load().armon_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "armon_memcpy_dtoh")
+        lib = self._load()
+        _check(lib.armon_synchronize(), "armon_synchronize", lib)
+        _check(lib.armon_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "armon_memcpy_dtoh", lib)
         return out
 
     def free(self):
