@@ -595,7 +595,7 @@ struct YColumn {
 
         // Projection of cell P = A-1 = n-R from faces P (h*[0]) and A (h*[1]).
         const int P = n - R;
-        if (P >= y0 && lane_ok) {
+        if (P >= y0 && P < y1 && lane_ok) {
             constexpr int QP = E2 ? 1 : 2;    // q index of cell P
             constexpr int FP = E2 ? 1 : 2;    // f_u index of face P
             T r = q_rho[QP], u = q_un[QP], v = q_ut[QP], e = q_E[QP];
@@ -614,19 +614,49 @@ struct YColumn {
 
 // One Y run: column x (a lane at or past nx marches benign cells and writes nothing), output
 // rows [y0, y1), with the cycle's dt. The whole wave must call it with the same y0, y1.
-template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD>
-__device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, int y0, int y1)
+//
+// SYNC: every wave of the block marches the same rows (of adjacent columns) and meets a block
+// barrier after every 4 rows, so that the block's loads and stores of a row stay within 4 rows
+// of each other. Waves that march independently drift apart by many rows, and the pieces that
+// each wave reads and writes per array and row (512 bytes in FP64) then fall in different DRAM
+// pages: a copy with the FP64 march's access pattern and no arithmetic moves 4.2 TB/s that way
+// and 4.6 TB/s with a barrier every 1-8 rows. The FP32-mixed march (92 VGPRs) is 8-11% faster
+// with the barrier. The FP64 march is not: the barrier raises it from 166 to 178 VGPRs (2 waves
+// per SIMD instead of 3; capped at 168 it spills), and both are 2-4% slower than without. `idle`
+// waves (past the last column) take part in the barriers only. SYNC needs the same rows for the
+// whole block; the batched kernels, whose waves march different runs, do not use it.
+template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD, bool SYNC = false>
+__device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, int y0, int y1, bool idle = false)
 {
     using Col = YColumn<T, FLUX, PROJ, EOS, DXM, STALE, AHEAD>;
     constexpr int R = Col::R;
-    Col col;
-    col.init(x, x < a.nx);
-    if constexpr (AHEAD) col.fetch(a, y0 - R);
     // a row counter from 0 keeps the loop and the row index in scalar registers (counting
     // from y0 - R, the compiler carried the row index in a vector register)
     const int rows = y1 - y0 + 2 * R;
+    // SYNC marches whole groups of 4 rows: up to 3 steps past the run, which load nothing (the
+    // row-ahead load stops at the run's last row; without stale layers a cell is made from the
+    // loaded state alone) and store nothing (outputs are rows [y0, y1) only)
+    static_assert(!(SYNC && STALE), "the block-synchronised march has no stale ghost layers");
+    const int groups = (rows + 3) / 4;
+    if constexpr (SYNC) {
+        if (idle) {
+            for (int g = 0; g < groups; ++g) __syncthreads();
+            return;
+        }
+    }
+    Col col;
+    col.init(x, x < a.nx);
+    if constexpr (AHEAD) col.fetch(a, y0 - R);
+    if constexpr (SYNC) {
+        for (int g = 0; g < groups; ++g) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + 4 * g + k, y0, y1);
+            __syncthreads();
+        }
+    } else {
 #pragma unroll kYUnroll
-    for (int i = 0; i < rows; ++i) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + i, y0, y1);
+        for (int i = 0; i < rows; ++i) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + i, y0, y1);
+    }
     if (a.reduce) reduce_cfl(a.st, col.local);
 }
 
@@ -638,10 +668,12 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     const int bid = xcd_block(blockIdx.x, gridDim.x);   // runs in segment-major order
     const int seg = bid / ncolblk;
     const int x = (bid - seg * ncolblk) * kYBlock + threadIdx.x;
-    if (x - (int) (threadIdx.x & 63) >= a.nx) return;   // a wave past the last column (no barriers follow)
+    const bool idle = x - (int) (threadIdx.x & 63) >= a.nx;   // a wave past the last column
     const int y0 = seg * a.seg_len;
     constexpr bool ahead = ARMON_Y_PREFETCH;
-    sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead>(a, dt, x, y0, min(y0 + a.seg_len, a.ny));
+    constexpr bool sync = !STALE && sizeof(T) == 4;   // FP32-mixed (see sweep_y_run)
+    if (idle && !sync) return;   // no barriers follow
+    sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead, sync>(a, dt, x, y0, min(y0 + a.seg_len, a.ny), idle);
 }
 
 // ---------------------------------------------------------------------------------------
