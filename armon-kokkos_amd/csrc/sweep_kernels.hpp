@@ -619,8 +619,8 @@ struct YColumn {
 // barrier after every 4 rows, so that the block's loads and stores of a row stay within 4 rows
 // of each other. Waves that march independently drift apart by many rows, and the pieces that
 // each wave reads and writes per array and row (512 bytes in FP64) then fall in different DRAM
-// pages: a copy with the FP64 march's access pattern and no arithmetic moves 4.2 TB/s that way
-// and 4.6 TB/s with a barrier every 1-8 rows. The FP32-mixed march (92 VGPRs) is 8-11% faster
+// pages: a copy with the FP64 march's access pattern and no arithmetic (ymarch_copy.hip) moves
+// 4.2 TB/s that way and 4.6 TB/s with a barrier every 1-8 rows. The FP32-mixed march (92 VGPRs) is 8-11% faster
 // with the barrier. The FP64 march is not: the barrier raises it from 166 to 178 VGPRs (2 waves
 // per SIMD instead of 3; capped at 168 it spills), and both are 2-4% slower than without. `idle`
 // waves (past the last column) take part in the barriers only. SYNC needs the same rows for the
