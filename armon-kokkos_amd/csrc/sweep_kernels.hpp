@@ -51,7 +51,7 @@ constexpr int kYBlock = 256;  // Y sweep: columns per block
 #define ARMON_Y_UNROLL 4
 #endif
 #ifndef ARMON_Y_WAVES_PER_EU
-#define ARMON_Y_WAVES_PER_EU 0
+#define ARMON_Y_WAVES_PER_EU 3   // the FP64 march's occupancy (168 VGPRs); 0: no hint
 #endif
 #ifndef ARMON_Y_PREFETCH
 #define ARMON_Y_PREFETCH 1
@@ -480,7 +480,6 @@ struct YColumn {
     T f_u[5], f_p[2];                     // final face flux: u* at faces F-4..F, p* at F-1..F (F = n - GAD)
     T q_rho[4], q_un[4], q_ut[4], q_E[4]; // post-update cells U-3..U (U = F - 1)
     T h0[2], h1[2], h2[2], h3[2];         // advection fluxes at faces A-1, A (A = U - E2)
-    T cw[R + 1];                          // sound speed of cells n-R..n
     T nx_rho, nx_u, nx_v, nx_E;           // the next row's state (AHEAD)
     int x;
     bool lane_ok;                         // false for lanes past the domain
@@ -498,8 +497,6 @@ struct YColumn {
         for (int i = 0; i < 4; i++) { q_rho[i] = T(1); q_un[i] = T(0); q_ut[i] = T(0); q_E[i] = T(1); }
         f_p[0] = f_p[1] = T(0);
         h0[0] = h0[1] = h1[0] = h1[1] = h2[0] = h2[1] = h3[0] = h3[1] = T(0);
-#pragma unroll
-        for (int i = 0; i <= R; i++) cw[i] = T(1);
         nx_rho = T(1); nx_u = T(0); nx_v = T(0); nx_E = T(1);
         local = DBL_MAX;
     }
@@ -515,8 +512,11 @@ struct YColumn {
         }
     }
 
-    // Row n of a run that outputs rows [y0, y1).
-    __device__ __forceinline__ void step(const SweepArgs<T>& a, T dt, T dx, T inv_dx, int n, int y0, int y1)
+    // Row n of a run that outputs rows [y0, y1). `ring` (LDS, 8 rows x the block's columns) keeps
+    // each row's sound speed until the CFL bound of its output, R rows later: off the arithmetic's
+    // dependency chains, and 8 VGPRs fewer than a register window.
+    __device__ __forceinline__ void step(const SweepArgs<T>& a, T dt, T dx, T inv_dx, int n, int y0, int y1,
+                                         T (*ring)[kYBlock])
     {
         Cell<T> cn;
         if constexpr (AHEAD) {
@@ -532,7 +532,7 @@ struct YColumn {
         pre[0] = pre[1];
         pre[1] = pre[2];
         pre[2] = cn;
-        push(cw, cn.c);
+        ring[n & 7][threadIdx.x] = cn.c;
 
         // Godunov at face n (cells n-1 | n).
         T gu, gp;
@@ -607,7 +607,8 @@ struct YColumn {
             a.out_v[idx] = u;
             a.out_E[idx] = e;
             if (a.reduce)
-                local = kmin(local, cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) v, (double) u, (double) cw[0], 1.0));
+                local = kmin(local, cfl_cell<double>(a.cfl_dx, a.cfl_dy, (double) v, (double) u,
+                                                     (double) ring[P & 7][threadIdx.x], 1.0));
         }
     }
 };
@@ -620,11 +621,12 @@ struct YColumn {
 // of each other. Waves that march independently drift apart by many rows, and the pieces that
 // each wave reads and writes per array and row (512 bytes in FP64) then fall in different DRAM
 // pages: a copy with the FP64 march's access pattern and no arithmetic (ymarch_copy.hip) moves
-// 4.2 TB/s that way and 4.6 TB/s with a barrier every 1-8 rows. The FP32-mixed march (92 VGPRs) is 8-11% faster
-// with the barrier. The FP64 march is not: the barrier raises it from 166 to 178 VGPRs (2 waves
-// per SIMD instead of 3; capped at 168 it spills), and both are 2-4% slower than without. `idle`
-// waves (past the last column) take part in the barriers only. SYNC needs the same rows for the
-// whole block; the batched kernels, whose waves march different runs, do not use it.
+// 4.2 TB/s that way and 4.6 TB/s with a barrier every 1-8 rows. With the barrier the FP32-mixed
+// march (98 VGPRs) is 9% faster and the FP64 march 3.6%; the FP64 march fits 3 waves per SIMD
+// with it only with the sound-speed window in LDS and the 3-wave occupancy hint (168 VGPRs,
+// against 178 with a register window). `idle` waves (past the last column) take part in the
+// barriers only. SYNC needs the same rows for the whole block; the batched kernels, whose waves
+// march different runs, do not use it.
 template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD, bool SYNC = false>
 __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, int y0, int y1, bool idle = false)
 {
@@ -645,17 +647,18 @@ __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, 
         }
     }
     Col col;
+    __shared__ T ring[8][kYBlock];
     col.init(x, x < a.nx);
     if constexpr (AHEAD) col.fetch(a, y0 - R);
     if constexpr (SYNC) {
         for (int g = 0; g < groups; ++g) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + 4 * g + k, y0, y1);
+            for (int k = 0; k < 4; ++k) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + 4 * g + k, y0, y1, ring);
             __syncthreads();
         }
     } else {
 #pragma unroll kYUnroll
-        for (int i = 0; i < rows; ++i) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + i, y0, y1);
+        for (int i = 0; i < rows; ++i) col.step(a, dt, (T) a.dx, (T) a.inv_dx, y0 - R + i, y0, y1, ring);
     }
     if (a.reduce) reduce_cfl(a.st, col.local);
 }
@@ -671,7 +674,7 @@ __global__ void __launch_bounds__(kYBlock) ARMON_Y_OCCUPANCY k_sweep_y(SweepArgs
     const bool idle = x - (int) (threadIdx.x & 63) >= a.nx;   // a wave past the last column
     const int y0 = seg * a.seg_len;
     constexpr bool ahead = ARMON_Y_PREFETCH;
-    constexpr bool sync = !STALE && sizeof(T) == 4;   // FP32-mixed (see sweep_y_run)
+    constexpr bool sync = !STALE;   // see sweep_y_run
     if (idle && !sync) return;   // no barriers follow
     sweep_y_run<T, FLUX, PROJ, EOS, DXM, STALE, ahead, sync>(a, dt, x, y0, min(y0 + a.seg_len, a.ny), idle);
 }
