@@ -625,8 +625,9 @@ struct YColumn {
 // march (98 VGPRs) is 9% faster and the FP64 march 3.6%; the FP64 march fits 3 waves per SIMD
 // with it only with the sound-speed window in LDS and the 3-wave occupancy hint (168 VGPRs,
 // against 178 with a register window). `idle` waves (past the last column) take part in the
-// barriers only. SYNC needs the same rows for the whole block; the batched kernels, whose waves
-// march different runs, do not use it.
+// barriers only. SYNC needs the same number of rows for the whole block: the loop-free batched
+// kernel takes it for a block whose 4 waves hold live runs of equal length (most blocks, as
+// the runs are listed longest first) and the independent march otherwise.
 template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD, bool SYNC = false>
 __device__ __forceinline__ void sweep_y_run(const SweepArgs<T>& a, T dt, int x, int y0, int y1, bool idle = false)
 {
@@ -783,22 +784,45 @@ __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)
 }
 
 // One wave per Y item (the launch configuration the autotuner usually picks): no item loop, so
-// the run body keeps the single-domain kernel's register budget.
+// the run body keeps the single-domain kernel's register budget. The work list is sorted by run
+// length and otherwise keeps each domain's runs in column order, so a block's 4 waves are mostly
+// adjacent columns of one run: when all 4 waves have active items of the same row count, the
+// block marches them with the block barrier of sweep_y_run (same trip count for every wave),
+// which keeps their accesses of a row together in DRAM.
 template <class T, int FLUX, int PROJ, int EOS, bool STALE>
 __global__ void __launch_bounds__(kYBlock) __attribute__((amdgpu_waves_per_eu(3)))
         k_batch_y1(BatchArgs<T> b, const BatchItemY* __restrict__ items, int nitems)
 {
     const int lane = threadIdx.x & 63;
     const int it = __builtin_amdgcn_readfirstlane((int) ((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-    if (it >= nitems) return;   // the whole wave
-    const BatchItemY item = items[it];
+    BatchItemY item{0, 0, 0, 0};
+    bool live = it < nitems;
+    if (live) {
+        item = items[it];
+        live = b.states[item.domain].active != 0;
+    }
+    bool sync = false;
+    if constexpr (!STALE) {
+        __shared__ int rows_of_wave[kYBlock / 64];
+        if (lane == 0) rows_of_wave[threadIdx.x >> 6] = live ? item.y1 - item.y0 : -1;
+        __syncthreads();
+        sync = true;
+#pragma unroll
+        for (int w = 0; w < kYBlock / 64; w++) sync = sync && rows_of_wave[w] == rows_of_wave[0] && rows_of_wave[w] >= 0;
+    }
+    if (!live) return;   // the whole wave; never in a synchronised block
     const DeviceState* st = b.states + item.domain;
-    if (!st->active) return;
     const SweepArgs<T> a = batch_domain_args(b, item.domain, 1);
     const int x = item.x0 + lane;
     constexpr bool ahead = ARMON_Y_PREFETCH;
-    sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, (T) (st->current_dt * b.dt_factor), x,
-                                                                   item.y0, item.y1);
+    const T dt = (T) (st->current_dt * b.dt_factor);
+    if constexpr (!STALE) {
+        if (sync) {
+            sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead, true>(a, dt, x, item.y0, item.y1);
+            return;
+        }
+    }
+    sweep_y_run<T, FLUX, PROJ, EOS, kBatchDxMode<T>, STALE, ahead>(a, dt, x, item.y0, item.y1);
 }
 
 // X tiles covering a stream of `slots` slots (tiles of W outputs overlapping by 2R): the last
