@@ -461,12 +461,11 @@ __device__ __forceinline__ void push(T (&w)[N], T v)
 // STALE = false, when the stencil width covers the read extent so that no ghost layer is stale,
 // the march then loads nothing else.
 // Whether the Y march shares divisors between Godunov's two quotients and the projection's three
-// (with_divisor). The X tiles do, which removes 3 of their 17 FP64 reciprocals per cell and 3%
-// of their time; in the march, which keeps 3 waves per SIMD and waits on its dependency chains
-// more than on instruction issue, the same change cut the VALU busy cycles by 4% without
-// shortening the sweep (the range checks and their scalar mask work lengthen the chains), so it
-// divides each quotient.
-constexpr bool kYSharedDivisor = false;
+// (with_divisor), as the X tiles do (3 of their 17 FP64 reciprocals per cell removed, 3% of
+// their time). While the march's waves ran independently it gained nothing (4% fewer VALU busy
+// cycles, the same time: the waves waited on their dependency chains); with the block barrier
+// every 4 rows it is 2.3% faster at 8192^2 and still fits 168 VGPRs with no spill.
+constexpr bool kYSharedDivisor = true;
 
 template <class T, int FLUX, int PROJ, int EOS, int DXM, bool STALE, bool AHEAD>
 struct YColumn {
