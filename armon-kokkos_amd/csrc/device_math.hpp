@@ -150,8 +150,39 @@ ARMON_HD void with_divisor_t(T b, Body&& body)
         body([&](T a) { return dv<T>(a, b); });
 }
 
+// The compiler's correctly rounded FP64 square root on gfx950 is rsq followed by two corrections
+// (sqrt_core's 10 operations), wrapped by a scaling (operands below 2^-767 are multiplied by
+// 2^256 and the root by 2^-128) and a final select that returns the operand itself for +-0 and
+// +inf: 21 instructions. For a finite operand of at least 2^-767 the wrappers leave every value
+// unchanged, so sqrt_t runs the 10 operations alone there, with the bits of the library square
+// root, and the library square root for every other operand (zero, subnormal, tiny, infinite,
+// negative or NaN), out of line.
+#if defined(__HIP_DEVICE_COMPILE__)
+ARMON_HD double sqrt_core(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+#endif
+
+ARMON_HD int sqrt_core_ok(double x) { return (int) (x >= 0x1p-767) & (int) (x < __builtin_huge_val()); }
+
 template <class T> ARMON_HD T sqrt_t(T x);
-template <> ARMON_HD double sqrt_t<double>(double x) { return __builtin_sqrt(x); }
+template <> ARMON_HD double sqrt_t<double>(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_expect(sqrt_core_ok(x), 1)) return sqrt_core(x);
+#endif
+    return __builtin_sqrt(x);
+}
 template <> ARMON_HD float sqrt_t<float>(float x) { return __builtin_sqrtf(x); }
 
 template <class T> ARMON_HD T copysign_t(T a, T b);
@@ -261,7 +292,7 @@ ARMON_HD void bizarrium(double rho, double E, double u, double v, double& p_out,
 
     double e = E - 0.5 * (u * u + v * v);
     double p = pk0 + G0 * rho0 * (e - eps_k0);
-    double c = __builtin_sqrt(G0 * rho0 * (p - pk0) - pk0prime) / rho;
+    double c = sqrt_t<double>(G0 * rho0 * (p - pk0) - pk0prime) / rho;
     double G0rho0 = G0 * rho0;
     g_out = 0.5 / (cr_pow3(rho) * (c * c)) * (pk0second + (G0rho0 * G0rho0) * (p - pk0));
     p_out = p;

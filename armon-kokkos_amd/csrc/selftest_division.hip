@@ -1,9 +1,10 @@
-// Self-test of the shared-divisor quotients (with_divisor in device_math.hpp): on the device,
-// three quotients over one divisor are compared bit for bit with the compiler's IEEE a / b, for
-// operands drawn over the whole FP64 range (subnormals, zeros of both signs, infinities, NaNs),
-// around the bounds of the ranges where the shared sequence is used, and in the range of
-// physical states. Built as its own library, libarmon_selftest.so; the solver library does not
-// contain it.
+// Self-tests of the shortened FP64 sequences of device_math.hpp, on the device: the
+// shared-divisor quotients (with_divisor: three quotients over one divisor against the
+// compiler's IEEE a / b) and the square root (sqrt_t<double> against the compiler's
+// __builtin_sqrt), bit for bit, for operands drawn over the whole FP64 range (subnormals, zeros
+// of both signs, infinities, NaNs), around the bounds of the ranges where the short sequences
+// are taken, and in the range of physical states. Built as its own library,
+// libarmon_selftest.so; the solver library does not contain it.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -114,13 +115,44 @@ __global__ void __launch_bounds__(256) k_check_shared_divisor(uint64_t seed, int
     if (fast) atomicAdd(&counts[1], fast);
 }
 
-}  // namespace
+__device__ double draw_radicand(uint64_t& s)
+{
+    const uint64_t r = splitmix64(s);
+    const bool neg = ((r >> 8) & 15) == 0;
+    const int span = (int) ((r >> 16) & 0xFFFF);
+    switch (r & 7) {
+    case 0: return with_exponent(s, -1075 + span % 2099, neg);   // every exponent
+    case 1: return with_exponent(s, -777 + span % 21, false);    // around 2^-767
+    case 2: return special(r >> 20);
+    case 3: return with_exponent(s, 1013 + span % 11, false);    // near the largest finite
+    default: return with_exponent(s, -30 + span % 61, false);    // physical states
+    }
+}
 
-// Checks n triples drawn from seed. mismatches: quotients that differ from a / b; shared:
-// triples that took the shared sequence; first_bad: b, a0, a1, a2 of one differing triple
-// (untouched when none differs). Returns 0, or the HIP error code.
-extern "C" __attribute__((visibility("default"))) int armon_selftest_shared_divisor(
-        uint64_t seed, int64_t n, uint64_t* mismatches, uint64_t* shared, double* first_bad)
+// counts[0]: roots whose bits differ from __builtin_sqrt; counts[1]: operands that took the
+// short sequence; first[0]: the operand of a differing root.
+__global__ void __launch_bounds__(256) k_check_sqrt(uint64_t seed, int64_t n, unsigned long long* counts,
+                                                     double* first, int* first_taken)
+{
+    unsigned long long bad = 0, fast = 0;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        uint64_t s = seed ^ ((uint64_t) i * 0xD1B54A32D192ED03ull);
+        const double x = draw_radicand(s);
+        const int miss = (int) (to_bits(armon::sqrt_t<double>(x)) != to_bits(__builtin_sqrt(x)));
+        bad += (unsigned long long) miss;
+        fast += (unsigned long long) armon::sqrt_core_ok(x);
+        if (miss && atomicCAS(first_taken, 0, 1) == 0) first[0] = x;
+    }
+    if (bad) atomicAdd(&counts[0], bad);
+    if (fast) atomicAdd(&counts[1], fast);
+}
+
+using CheckKernel = void (*)(uint64_t, int64_t, unsigned long long*, double*, int*);
+
+// Runs one check kernel over n samples and copies its two counts and, when a sample failed,
+// its nfirst operands back. Returns 0, or the HIP error code.
+int run_check(CheckKernel kernel, uint64_t seed, int64_t n, uint64_t* mismatches, uint64_t* taken_short,
+              double* first_bad, int nfirst)
 {
     unsigned long long* counts = nullptr;
     double* first = nullptr;
@@ -131,7 +163,7 @@ extern "C" __attribute__((visibility("default"))) int armon_selftest_shared_divi
     if (e == hipSuccess) e = hipMemset(counts, 0, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(taken, 0, sizeof(int));
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_check_shared_divisor, dim3(2048), dim3(256), 0, 0, seed, n, counts, first, taken);
+        hipLaunchKernelGGL(kernel, dim3(2048), dim3(256), 0, 0, seed, n, counts, first, taken);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -139,13 +171,34 @@ extern "C" __attribute__((visibility("default"))) int armon_selftest_shared_divi
     int t = 0;
     if (e == hipSuccess) e = hipMemcpy(h, counts, sizeof(h), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(&t, taken, sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && t) e = hipMemcpy(first_bad, first, 4 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && t) e = hipMemcpy(first_bad, first, nfirst * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess) {
         *mismatches = h[0];
-        *shared = h[1];
+        *taken_short = h[1];
     }
     (void) hipFree(counts);
     (void) hipFree(first);
     (void) hipFree(taken);
     return (int) e;
+}
+
+}  // namespace
+
+// Checks n triples drawn from seed. mismatches: quotients that differ from a / b; shared:
+// triples that took the shared sequence; first_bad: b, a0, a1, a2 of one differing triple
+// (untouched when none differs). Returns 0, or the HIP error code.
+extern "C" __attribute__((visibility("default"))) int armon_selftest_shared_divisor(
+        uint64_t seed, int64_t n, uint64_t* mismatches, uint64_t* shared, double* first_bad)
+{
+    return run_check(k_check_shared_divisor, seed, n, mismatches, shared, first_bad, 4);
+}
+
+// Checks n square roots drawn from seed. mismatches: roots that differ from __builtin_sqrt;
+// fast: operands that took the short sequence; first_bad: the operand of one differing root
+// (untouched when none differs). Returns 0, or the HIP error code.
+extern "C" __attribute__((visibility("default"))) int armon_selftest_sqrt(uint64_t seed, int64_t n,
+                                                                          uint64_t* mismatches, uint64_t* fast,
+                                                                          double* first_bad)
+{
+    return run_check(k_check_sqrt, seed, n, mismatches, fast, first_bad, 1);
 }
