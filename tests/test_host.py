@@ -51,6 +51,16 @@ def test_float_library_exports_the_kernel_abi(armon):
     assert lib.flt_size() == 4 and lib.idx_size() == 8 and lib.is_uidx_signed() is False
 
 
+def test_selftest_library_is_separate_from_the_solver(armon):
+    """lib/libarmon_selftest.so holds the device self-tests of the shortened FP64 sequences (the
+    shared-divisor quotients and the square root); the solver library exports neither."""
+    path = os.path.join(ROOT, "armon-kokkos_amd", "lib", "libarmon_selftest.so")
+    selftest = C.CDLL(path)
+    for name in ("armon_selftest_shared_divisor", "armon_selftest_sqrt"):
+        assert hasattr(selftest, name), name
+        assert not hasattr(armon.load(), name), name
+
+
 def test_abi_queries(armon):
     lib = armon.load()
     assert lib.idx_size() == 8
